@@ -1,0 +1,90 @@
+// pstm_common.h -- shared host/device definitions of the gfx950 PSTM library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pstm_abi.h"
+
+namespace pstm {
+
+constexpr uint64_t kPlanMagic = 0x50535450'4C414E31ull;  // "PSTPLAN1"
+
+// Forward/adjoint tile: TILE_TAU output times per workgroup (one per thread),
+// a TX x TY block of image columns held in registers by every thread.
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int kTileTau = kThreads;
+
+// LDS capacities of the tile kernels.
+constexpr int kSampleCap = 6144;  // staged trace samples (floats) per sub-batch
+constexpr int kMaxBatch = 64;     // traces per sub-batch
+constexpr int kQueueCap = 320;    // candidate queue entries (>= kMaxBatch - 1 + kThreads)
+
+// Margin (samples) added on each side of a staged window beyond the
+// rigorous traveltime bounds (absorbs FP32 rounding of the bounds).
+constexpr int kWindowMargin = 2;
+
+// Plan layout: everything the kernels need about the trace binning, derived
+// deterministically from the job parameters on the host.
+struct PlanLayout {
+  int64_t n_traces;
+  int32_t ncx, ncy;      // spatial cells (along il, along xl)
+  int32_t key_bits;      // radix-sort key width
+  float cell_x0, cell_y0;  // local coordinate of the cell grid corner (m)
+  float cell_w, cell_h;    // cell size (m)
+  float d_il, d_xl;        // column spacing (m)
+  int32_t tile_x, tile_y;  // columns per tile along il / xl
+  double ux_x, ux_y;       // unit vector of the inline axis (world)
+  double uy_x, uy_y;       // unit vector of the crossline axis (world)
+  // byte offsets inside the workspace
+  size_t off_header, off_keys_in, off_keys_out, off_ids_in, off_ids_out;
+  size_t off_local, off_geom, off_cell_start, off_sort_tmp;
+  size_t sort_tmp_bytes;
+  size_t total_bytes;
+};
+
+// Device-resident header at workspace offset 0.
+struct PlanHeader {
+  uint64_t magic;
+  int64_t n_traces;
+  int64_t n_kept;
+  int32_t ncx, ncy;
+};
+
+// Arguments of the tile kernels (forward / adjoint / velocity gradient).
+struct TileArgs {
+  // sizes
+  int32_t n_samples, n_il, n_xl, n_tau;
+  int32_t n_tiles_x, n_tiles_y, n_tblocks;
+  // time axes
+  float inv_dt, t0_over_dt, umax, u_last;  // umax = n-1.001, u_last = n-1
+  float tau0, dtau;
+  // geometry
+  float d_il, d_xl;
+  float aperture, tan_th;
+  float taper_width;
+  float aa_scale_over_dt, aa_max;
+  int32_t aa_pad;           // staged zero padding on each side (samples)
+  int32_t flags, weight_mode, vel_mode;
+  // plan
+  int32_t ncx, ncy;
+  float cell_x0, cell_y0, cell_w, cell_h;
+  const float4* geom;        // sorted local (sa, sb, ra, rb)
+  const int32_t* ids;        // sorted -> original trace index
+  const int32_t* cell_start; // [ncx*ncy + 1]
+  // data
+  const void* traces;
+  const float* scale;
+  const float* vel;
+  const float* eta;
+  float* image;
+  float* comp;
+  float* fold;
+  // adjoint / gradient
+  const float* image_in;     // adjoint input image or gradient upstream
+  float* traces_out;         // adjoint output
+  float* vel_grad;           // gradient output
+};
+
+}  // namespace pstm

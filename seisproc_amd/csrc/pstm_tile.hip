@@ -1,0 +1,642 @@
+// pstm_tile.hip -- gfx950 Kirchhoff PSTM tile kernels (forward, adjoint,
+// velocity gradient).
+//
+// Work decomposition (one workgroup = 4 wave64 = 256 threads):
+//   * a tile of TX x TY image columns and 256 consecutive output times;
+//     thread t owns output time tau_t for ALL columns of the tile, so the
+//     per-column accumulators live in registers and the image is written
+//     once per tile, coalesced along tau, without atomics;
+//   * the traces that can reach the tile are streamed from the plan's
+//     spatial cells (a few contiguous ranges of the sorted trace list),
+//     filtered against the tile's aperture/angle reach, and compacted in
+//     order (deterministic summation order) into an LDS queue;
+//   * each queued trace gets a sample window bounded rigorously from the DSR
+//     traveltime over the tile's columns and times; sub-batches of windows
+//     are staged HBM -> LDS with coalesced loads (FP32/BF16/FP16 converted to
+//     FP32 with the per-trace scale applied), integrated twice in LDS when
+//     the triangle antialias filter is on;
+//   * per (trace, column) the source/receiver horizontal distances are
+//     computed once into LDS; each lane then evaluates the DSR time, masks,
+//     weight and interpolation for its own output time.
+//
+// Reference method: processors/migration/kirchhoff_kernel.py:459-854
+// (migrate_kirchhoff_full) -- same masks (:635, :670-685), DSR time (:688),
+// weight (:691-693), clamped linear interpolation (:708-721), image and fold
+// sums (:724-730).  The traveltime is evaluated in the equivalent
+// time form t_leg = sqrt(tau^2/4 + h^2 / v^2), which is r/v with
+// z = v tau / 2 (migration_engine.py:305-306).
+#include "pstm_internal.h"
+
+namespace pstm {
+
+enum { MODE_FWD = 0, MODE_ADJ = 1, MODE_GRAD = 2 };
+
+__device__ __forceinline__ float wave_max(float v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+  return v;
+}
+__device__ __forceinline__ float wave_min(float v) {
+  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off));
+  return v;
+}
+__device__ __forceinline__ float uniform_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Inclusive prefix sum over the 64 lanes of a wave.
+__device__ __forceinline__ float wave_scan_f(float v, int lane) {
+  for (int off = 1; off < 64; off <<= 1) {
+    float o = __shfl_up(v, off);
+    if (lane >= off) v += o;
+  }
+  return v;
+}
+__device__ __forceinline__ int wave_scan_i(int v, int lane) {
+  for (int off = 1; off < 64; off <<= 1) {
+    int o = __shfl_up(v, off);
+    if (lane >= off) v += o;
+  }
+  return v;
+}
+
+template <int DT>
+__device__ __forceinline__ float load_sample(const void* tr, int64_t idx) {
+  if constexpr (DT == PSTM_F32) {
+    return ((const float*)tr)[idx];
+  } else if constexpr (DT == PSTM_BF16) {
+    uint32_t b = ((const uint16_t*)tr)[idx];
+    return __uint_as_float(b << 16);
+  } else {
+    uint16_t b = ((const uint16_t*)tr)[idx];
+    return (float)(*(const _Float16*)&b);
+  }
+}
+
+// Neumaier two-term update: (s, c) += y.
+__device__ __forceinline__ void neumaier(float& s, float& c, float y) {
+  float t = s + y;
+  c += (fabsf(s) >= fabsf(y)) ? ((s - t) + y) : ((y - t) + s);
+  s = t;
+}
+
+// Cosine taper on rho = h / limit (weights.py:192-236 shape).
+__device__ __forceinline__ float taper_of(float hm2, float hl2, float tw) {
+  float rho = sqrtf(hm2 / hl2);
+  float start = 1.0f - tw;
+  if (rho <= start) return 1.0f;
+  float x = (rho - start) / tw;
+  return 0.5f * (1.0f + __cosf(3.14159265358979f * fminf(x, 1.0f)));
+}
+
+struct TileShared {
+  float samp[kSampleCap];
+  int32_t q_j[kQueueCap];
+  int32_t q_w0[kQueueCap];
+  int32_t q_len[kQueueCap];
+  int32_t b_base[kMaxBatch];   // LDS index of sample 0 of the trace (off - w0)
+  int32_t b_off[kMaxBatch];    // LDS offset of the window
+  int32_t scan[kWaves];
+  int32_t queue_n;
+  int32_t nb;
+  float hwave[kWaves];          // max hlim^2 of each wave (over its columns and times)
+  float qmin[kWaves], qmax[kWaves], tqa[kWaves], tqb[kWaves];
+  float emax;                   // max eta over the tile (VTI)
+};
+
+template <int NC, bool AA>
+struct PairShared {
+  float2 h2[kMaxBatch][NC];
+  float4 g[AA ? kMaxBatch : 1][AA ? NC : 1];
+};
+
+template <int MODE, int TX, int TY, int DT, bool AA, bool VTI>
+__global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
+  constexpr int NC = TX * TY;
+  __shared__ TileShared S;
+  __shared__ PairShared<NC, AA> P;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+  // ---- block -> (time block, tile): deepest time blocks first, each XCD a
+  // contiguous spatial chunk of tiles.
+  const int n_tiles = a.n_tiles_x * a.n_tiles_y;
+  const int chunk = (n_tiles + 7) / 8;
+  const int xcd = blockIdx.x & 7, r = blockIdx.x >> 3;
+  const int tb = a.n_tblocks - 1 - r / chunk;
+  const int tile = xcd * chunk + r % chunk;
+  if (tile >= n_tiles || tb < 0) return;
+  const int tx = tile / a.n_tiles_y, ty = tile % a.n_tiles_y;
+
+  // ---- per-lane constants
+  const int k = tb * kTileTau + tid;
+  const bool kvalid = k < a.n_tau;
+  const int kk = kvalid ? k : a.n_tau - 1;
+  const float tau = a.tau0 + (float)kk * a.dtau;
+  const float tq = 0.25f * tau * tau;
+  const float tqx = fmaxf(tq, 1e-12f);
+
+  float q[NC], hl2[NC];
+  [[maybe_unused]] float k1[VTI ? NC : 1], m2[VTI ? NC : 1];
+  float emax_l = 0.f;
+  float qmin_l = 3.0e38f, qmax_l = 0.f, hmax_l = -1.f;
+  float colX[NC], colY[NC];
+  bool cval[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int il = tx * TX + c / TY, xl = ty * TY + c % TY;
+    cval[c] = il < a.n_il && xl < a.n_xl;
+    colX[c] = (float)il * a.d_il;
+    colY[c] = (float)xl * a.d_xl;
+    const int64_t vi = a.vel_mode == PSTM_VEL_1D
+                           ? (int64_t)kk
+                           : ((int64_t)(cval[c] ? il : 0) * a.n_xl + (cval[c] ? xl : 0)) * a.n_tau + kk;
+    const float v = a.vel[vi];
+    const bool ok = kvalid && cval[c] && v > 0.f;
+    const float vv = ok ? v : 1.f;
+    q[c] = 1.0f / (vv * vv);
+    const float z = 0.5f * vv * tau;
+    float ap = a.aperture;
+    if (a.flags & PSTM_FLAG_TIME_APERTURE) ap = fminf(ap, z * a.tan_th);
+    const float hl = fminf(ap, a.tan_th * (z + 1e-6f));
+    hl2[c] = ok ? hl * hl : -1.f;
+    if constexpr (VTI) {
+      const float e = ok ? a.eta[vi] : 0.f;
+      k1[c] = 1.0f + 2.0f * e;
+      m2[c] = -2.0f * e;
+      emax_l = fmaxf(emax_l, e);
+    }
+    if (ok) {
+      qmin_l = fminf(qmin_l, q[c]);
+      qmax_l = fmaxf(qmax_l, q[c]);
+    }
+    hmax_l = fmaxf(hmax_l, hl2[c]);
+  }
+  // wave-uniform per-column reach (max over the wave's times)
+  float hcol[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) hcol[c] = uniform_f(wave_max(hl2[c]));
+  {
+    const float hw = wave_max(hmax_l), qn = wave_min(qmin_l), qx = wave_max(qmax_l);
+    const float em = wave_max(emax_l);
+    if (lane == 0) {
+      S.hwave[wave] = hw;
+      S.qmin[wave] = qn;
+      S.qmax[wave] = qx;
+      const int ka = min(tb * kTileTau + wave * 64, a.n_tau - 1);
+      const int kb = min(tb * kTileTau + wave * 64 + 63, a.n_tau - 1);
+      const float ta = a.tau0 + (float)ka * a.dtau, tbv = a.tau0 + (float)kb * a.dtau;
+      S.tqa[wave] = 0.25f * ta * ta;
+      S.tqb[wave] = 0.25f * tbv * tbv;
+      if (wave == 0) S.emax = 0.f;
+      if (wave == 0) S.queue_n = 0;
+    }
+    if constexpr (VTI) {
+      __syncthreads();
+      if (lane == 0) atomicMax((int*)&S.emax, __float_as_int(em));  // em >= 0
+    }
+  }
+  __syncthreads();
+  float hblk = -1.f;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) hblk = fmaxf(hblk, S.hwave[w]);
+  const float emax = VTI ? S.emax : 0.f;
+
+  // ---- accumulators
+  float acc[NC], accb[NC], cmp[NC];
+  int fcnt[NC];
+  [[maybe_unused]] float m_in[MODE == MODE_ADJ ? NC : 1];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    acc[c] = 0.f;
+    accb[c] = 0.f;
+    cmp[c] = 0.f;
+    fcnt[c] = 0;
+  }
+  if constexpr (MODE == MODE_ADJ) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int il = tx * TX + c / TY, xl = ty * TY + c % TY;
+      m_in[c] = (kvalid && cval[c]) ? a.image_in[((int64_t)il * a.n_xl + xl) * a.n_tau + k] : 0.f;
+    }
+  }
+
+  if (hblk > 0.f) {
+    // tile rectangle over valid columns (local coordinates)
+    const int il_hi = min(tx * TX + TX - 1, a.n_il - 1), xl_hi = min(ty * TY + TY - 1, a.n_xl - 1);
+    const float X0 = (float)(tx * TX) * a.d_il, X1 = (float)il_hi * a.d_il;
+    const float Y0 = (float)(ty * TY) * a.d_xl, Y1 = (float)xl_hi * a.d_xl;
+    const float R = sqrtf(hblk);
+    int cx0 = (int)floorf((X0 - R - a.cell_x0) / a.cell_w);
+    int cx1 = (int)floorf((X1 + R - a.cell_x0) / a.cell_w);
+    int cy0 = (int)floorf((Y0 - R - a.cell_y0) / a.cell_h);
+    int cy1 = (int)floorf((Y1 + R - a.cell_y0) / a.cell_h);
+    cx0 = max(cx0, 0);
+    cy0 = max(cy0, 0);
+    cx1 = min(cx1, a.ncx - 1);
+    cy1 = min(cy1, a.ncy - 1);
+    const int pad = AA ? a.aa_pad : 0;
+    const int wlo = -pad - kWindowMargin, whi = a.n_samples - 1 + pad + kWindowMargin;
+
+    // Process the queued traces in LDS-sized sub-batches.  `final_flush`:
+    // also process a last partial batch.
+    auto process_queue = [&](bool final_flush) {
+      int qs = 0;
+      const int qn = S.queue_n;
+      while (true) {
+        const int rem = qn - qs;
+        if (rem <= 0 || (!final_flush && rem < kMaxBatch)) break;
+        if (wave == 0) {
+          const int i = qs + lane;
+          const bool in = lane < kMaxBatch && i < qn;
+          const int len = in ? S.q_len[i] : 0;
+          const int incl = wave_scan_i(len, lane);
+          const bool fit = in && incl <= kSampleCap;
+          const uint64_t m = __ballot(fit);
+          if (fit) {
+            S.b_off[lane] = incl - len;
+            S.b_base[lane] = incl - len - S.q_w0[i];
+          }
+          if (lane == 0) S.nb = __popcll(m);
+        }
+        __syncthreads();
+        const int nb = S.nb;
+
+        // -- stage (FWD/GRAD: load samples; ADJ: zero the windows)
+        for (int b = wave; b < nb; b += kWaves) {
+          const int e = qs + b;
+          const int w0 = S.q_w0[e], len = S.q_len[e], off = S.b_off[b];
+          if constexpr (MODE == MODE_ADJ) {
+            for (int s = lane; s < len; s += 64) S.samp[off + s] = 0.f;
+          } else {
+            const int id = a.ids[S.q_j[e]];
+            const float sc = a.scale ? a.scale[id] : 1.f;
+            const int64_t row = (int64_t)id * a.n_samples;
+            for (int s = lane; s < len; s += 64) {
+              const int src = w0 + s;
+              float v = 0.f;
+              if (src >= 0 && src < a.n_samples) v = load_sample<DT>(a.traces, row + src) * sc;
+              S.samp[off + s] = v;
+            }
+          }
+        }
+        // -- (trace, column) distances
+        for (int idx = tid; idx < nb * NC; idx += kThreads) {
+          const int b = idx / NC, c = idx % NC;
+          const float4 g = a.geom[S.q_j[qs + b]];
+          const float X = (float)(tx * TX + c / TY) * a.d_il, Y = (float)(ty * TY + c % TY) * a.d_xl;
+          const float dxs = X - g.x, dys = Y - g.y, dxr = X - g.z, dyr = Y - g.w;
+          P.h2[b][c] = make_float2(fmaf(dxs, dxs, dys * dys), fmaf(dxr, dxr, dyr * dyr));
+          if constexpr (AA) P.g[b][c] = make_float4(dxs, dys, dxr, dyr);
+        }
+        __syncthreads();
+        if constexpr (AA && MODE != MODE_ADJ) {
+          // double integration (causal, then anticausal) of each window
+          for (int b = wave; b < nb; b += kWaves) {
+            const int len = S.q_len[qs + b], off = S.b_off[b];
+            float carry = 0.f;
+            for (int s0 = 0; s0 < len; s0 += 64) {
+              const int s = s0 + lane;
+              float v = s < len ? S.samp[off + s] : 0.f;
+              v = wave_scan_f(v, lane) + carry;
+              if (s < len) S.samp[off + s] = v;
+              carry = __shfl(v, 63);
+            }
+            carry = 0.f;
+            for (int s0 = 0; s0 < len; s0 += 64) {
+              const int s = len - 1 - (s0 + lane);
+              float v = s >= 0 ? S.samp[off + s] : 0.f;
+              v = wave_scan_f(v, lane) + carry;
+              if (s >= 0) S.samp[off + s] = v;
+              carry = __shfl(v, 63);
+            }
+          }
+          __syncthreads();
+        }
+
+        // -- contributions
+        for (int b = 0; b < nb; ++b) {
+          const int base = uniform_i(S.b_base[b]);
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            const float2 h2 = P.h2[b][c];
+            const float hm = uniform_f(fmaxf(h2.x, h2.y));
+            if (!(hm < hcol[c])) continue;
+            const bool pass = hm < hl2[c];
+            const float xs = fmaf(h2.x, q[c], tqx), xr = fmaf(h2.y, q[c], tqx);
+            const float rs = __builtin_amdgcn_rsqf(xs), rr = __builtin_amdgcn_rsqf(xr);
+            float t;
+            if constexpr (VTI) {
+              const float ps = h2.x * q[c], pr = h2.y * q[c];
+              const float ds = fmaf(k1[c], ps, tqx), dr = fmaf(k1[c], pr, tqx);
+              const float vs = fmaf(m2[c] * ps, ps * __builtin_amdgcn_rcpf(ds), tqx + ps);
+              const float vr = fmaf(m2[c] * pr, pr * __builtin_amdgcn_rcpf(dr), tqx + pr);
+              t = __builtin_amdgcn_sqrtf(fmaxf(vs, 1e-12f)) + __builtin_amdgcn_sqrtf(fmaxf(vr, 1e-12f));
+            } else {
+              t = fmaf(xs, rs, xr * rr);
+            }
+            const float u_raw = fmaf(t, a.inv_dt, -a.t0_over_dt);
+            bool ok = pass;
+            if (a.flags & PSTM_FLAG_ZERO_OUTSIDE) ok = ok && (u_raw >= 0.f) && (u_raw <= a.u_last);
+            const float u = __builtin_amdgcn_fmed3f(u_raw, 0.f, a.umax);
+            const float prs = rs * rr;
+            float w;
+            if (a.weight_mode == PSTM_WEIGHT_FULL) w = (tq * q[c]) * prs * prs;
+            else if (a.weight_mode == PSTM_WEIGHT_SPREADING) w = q[c] * prs;
+            else w = 1.f;
+            if (a.flags & PSTM_FLAG_TAPER) w *= taper_of(hm, hl2[c], a.taper_width);
+            float L = 1.f, iL2 = 1.f;
+            if constexpr (AA) {
+              const float4 g = P.g[b][c];
+              const float ga = fmaf(g.x, rs, g.z * rr), gb = fmaf(g.y, rs, g.w * rr);
+              L = q[c] * a.aa_scale_over_dt * fmaf(a.d_il, fabsf(ga), a.d_xl * fabsf(gb));
+              L = __builtin_amdgcn_fmed3f(L, 1.f, a.aa_max);
+              iL2 = __builtin_amdgcn_rcpf(L * L);
+            }
+            if constexpr (MODE == MODE_FWD || MODE == MODE_GRAD) {
+              const int i0 = (int)u;
+              const float f = u - (float)i0;
+              const float a0 = S.samp[base + i0], a1 = S.samp[base + i0 + 1];
+              float amp = fmaf(f, a1 - a0, a0);
+              if constexpr (AA) {
+                const float um = u - L, up = u + L;
+                const int im = (int)floorf(um), ip = (int)up;
+                const float fm = um - (float)im, fp = up - (float)ip;
+                const float b0 = S.samp[base + im], b1 = S.samp[base + im + 1];
+                const float c0 = S.samp[base + ip], c1 = S.samp[base + ip + 1];
+                const float am = fmaf(fm, b1 - b0, b0), ap2 = fmaf(fp, c1 - c0, c0);
+                amp = (2.f * amp - am - ap2) * iL2;
+              }
+              if constexpr (MODE == MODE_FWD) {
+                accb[c] += ok ? w * amp : 0.f;
+                fcnt[c] += ok ? 1 : 0;
+              } else {
+                // d/dv of w * amp with masks held (isotropic, no antialias)
+                const float sqv = __builtin_amdgcn_sqrtf(q[c]);  // 1/v
+                const float hsr = h2.x * rs * rs, hrr = h2.y * rr * rr;
+                float dw;
+                if (a.weight_mode == PSTM_WEIGHT_FULL) dw = -2.f * sqv * w * (1.f - q[c] * (hsr + hrr));
+                else if (a.weight_mode == PSTM_WEIGHT_SPREADING)
+                  dw = -2.f * sqv * w * (1.f - 0.5f * q[c] * (hsr + hrr));
+                else dw = 0.f;
+                const bool clamped = !(u_raw >= 0.f && u_raw <= a.umax);
+                const float dt_dv = -sqv * q[c] * fmaf(h2.x, rs, h2.y * rr);
+                const float slope = clamped ? 0.f : (a1 - a0);
+                const float d = fmaf(w * slope, dt_dv * a.inv_dt, dw * amp);
+                accb[c] += ok ? d : 0.f;
+              }
+            } else {
+              // adjoint: scatter w * m into the window (transpose of the taps)
+              const float gsc = ok ? w * m_in[c] : 0.f;
+              if (gsc != 0.f) {
+                if constexpr (AA) {
+                  const float um = u - L, up = u + L;
+                  const int i0 = (int)u, im = (int)floorf(um), ip = (int)up;
+                  const float f = u - (float)i0, fm = um - (float)im, fp = up - (float)ip;
+                  const float gc = 2.f * gsc * iL2, ge = -gsc * iL2;
+                  atomicAdd(&S.samp[base + i0], gc * (1.f - f));
+                  atomicAdd(&S.samp[base + i0 + 1], gc * f);
+                  atomicAdd(&S.samp[base + im], ge * (1.f - fm));
+                  atomicAdd(&S.samp[base + im + 1], ge * fm);
+                  atomicAdd(&S.samp[base + ip], ge * (1.f - fp));
+                  atomicAdd(&S.samp[base + ip + 1], ge * fp);
+                } else {
+                  const int i0 = (int)u;
+                  const float f = u - (float)i0;
+                  atomicAdd(&S.samp[base + i0], gsc * (1.f - f));
+                  atomicAdd(&S.samp[base + i0 + 1], gsc * f);
+                }
+              }
+            }
+          }
+        }
+        if constexpr (MODE != MODE_ADJ) {
+          if (a.flags & PSTM_FLAG_KAHAN) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+              neumaier(acc[c], cmp[c], accb[c]);
+              accb[c] = 0.f;
+            }
+          } else {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+              acc[c] += accb[c];
+              accb[c] = 0.f;
+            }
+          }
+        }
+        __syncthreads();
+        if constexpr (MODE == MODE_ADJ) {
+          if constexpr (AA) {
+            // transpose of the double integration: the same symmetric operator
+            for (int b = wave; b < nb; b += kWaves) {
+              const int len = S.q_len[qs + b], off = S.b_off[b];
+              float carry = 0.f;
+              for (int s0 = 0; s0 < len; s0 += 64) {
+                const int s = s0 + lane;
+                float v = s < len ? S.samp[off + s] : 0.f;
+                v = wave_scan_f(v, lane) + carry;
+                if (s < len) S.samp[off + s] = v;
+                carry = __shfl(v, 63);
+              }
+              carry = 0.f;
+              for (int s0 = 0; s0 < len; s0 += 64) {
+                const int s = len - 1 - (s0 + lane);
+                float v = s >= 0 ? S.samp[off + s] : 0.f;
+                v = wave_scan_f(v, lane) + carry;
+                if (s >= 0) S.samp[off + s] = v;
+                carry = __shfl(v, 63);
+              }
+            }
+          }
+          // flush the windows into the output traces
+          for (int b = wave; b < nb; b += kWaves) {
+            const int e = qs + b;
+            const int w0 = S.q_w0[e], len = S.q_len[e], off = S.b_off[b];
+            const int id = a.ids[S.q_j[e]];
+            float* row = a.traces_out + (int64_t)id * a.n_samples;
+            for (int s = lane; s < len; s += 64) {
+              const int dst = w0 + s;
+              const float v = S.samp[off + s];
+              if (dst >= 0 && dst < a.n_samples && v != 0.f) atomicAdd(&row[dst], v);
+            }
+          }
+          __syncthreads();
+        }
+        qs += nb;
+      }
+      // move the unprocessed tail (< kMaxBatch entries) to the front
+      const int rem = qn - qs;
+      int j = 0, w0 = 0, len = 0;
+      if (tid < rem) {
+        j = S.q_j[qs + tid];
+        w0 = S.q_w0[qs + tid];
+        len = S.q_len[qs + tid];
+      }
+      __syncthreads();
+      if (tid < rem) {
+        S.q_j[tid] = j;
+        S.q_w0[tid] = w0;
+        S.q_len[tid] = len;
+      }
+      if (tid == 0) S.queue_n = rem > 0 ? rem : 0;
+      __syncthreads();
+    };
+
+    for (int cy = cy0; cy <= cy1; ++cy) {
+      const int r0 = a.cell_start[cy * a.ncx + cx0];
+      const int r1 = a.cell_start[cy * a.ncx + cx1 + 1];
+      for (int j0 = r0; j0 < r1; j0 += kThreads) {
+        const int j = j0 + tid;
+        bool pass = false;
+        int w0 = 0, len = 0;
+        if (j < r1) {
+          const float4 g = a.geom[j];
+          // distances from source / receiver to the tile rectangle
+          const float sdx = fmaxf(fmaxf(X0 - g.x, g.x - X1), 0.f), sdy = fmaxf(fmaxf(Y0 - g.y, g.y - Y1), 0.f);
+          const float rdx = fmaxf(fmaxf(X0 - g.z, g.z - X1), 0.f), rdy = fmaxf(fmaxf(Y0 - g.w, g.w - Y1), 0.f);
+          const float hs_min = fmaf(sdx, sdx, sdy * sdy), hr_min = fmaf(rdx, rdx, rdy * rdy);
+          if (hs_min < hblk && hr_min < hblk) {
+            const float sfx = fmaxf(fabsf(g.x - X0), fabsf(g.x - X1)), sfy = fmaxf(fabsf(g.y - Y0), fabsf(g.y - Y1));
+            const float rfx = fmaxf(fabsf(g.z - X0), fabsf(g.z - X1)), rfy = fmaxf(fabsf(g.w - Y0), fabsf(g.w - Y1));
+            const float hs_max = fmaf(sfx, sfx, sfy * sfy), hr_max = fmaf(rfx, rfx, rfy * rfy);
+            const float ecorr = 1.0f / (1.0f + 2.0f * emax);
+            float tlo = 3.0e38f, thi = -1.f;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+              const float hw = S.hwave[w];
+              if (hs_min < hw && hr_min < hw) {
+                const float ta = fmaxf(S.tqa[w], 1e-12f), tbq = S.tqb[w] + 1e-12f;
+                const float lo = sqrtf(ta + hs_min * S.qmin[w] * ecorr) + sqrtf(ta + hr_min * S.qmin[w] * ecorr);
+                const float hi = sqrtf(tbq + hs_max * S.qmax[w]) + sqrtf(tbq + hr_max * S.qmax[w]);
+                tlo = fminf(tlo, lo);
+                thi = fmaxf(thi, hi);
+              }
+            }
+            if (thi >= 0.f) {
+              const float ul = __builtin_amdgcn_fmed3f(fmaf(tlo, a.inv_dt, -a.t0_over_dt), 0.f, a.umax);
+              const float uh = __builtin_amdgcn_fmed3f(fmaf(thi, a.inv_dt, -a.t0_over_dt), 0.f, a.umax);
+              int lo_i = (int)ul - pad - kWindowMargin;
+              int hi_i = (int)uh + 1 + pad + kWindowMargin;
+              lo_i = max(lo_i, AA ? wlo : 0);
+              hi_i = min(hi_i, AA ? whi : a.n_samples - 1);
+              w0 = lo_i;
+              len = hi_i - lo_i + 1;
+              pass = true;
+            }
+          }
+        }
+        // ordered compaction into the queue
+        const uint64_t m = __ballot(pass);
+        const int below = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) S.scan[wave] = __popcll(m);
+        __syncthreads();
+        int pre = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+          const int cnt = S.scan[w];
+          pre += w < wave ? cnt : 0;
+          tot += cnt;
+        }
+        const int qn = S.queue_n;
+        if (pass) {
+          const int slot = qn + pre + below;
+          S.q_j[slot] = j;
+          S.q_w0[slot] = w0;
+          S.q_len[slot] = len;
+        }
+        __syncthreads();
+        if (tid == 0) S.queue_n = qn + tot;
+        __syncthreads();
+        if (qn + tot >= kMaxBatch) process_queue(false);
+      }
+    }
+    process_queue(true);
+  }
+
+  // ---- write-out
+  if (!kvalid) return;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (!cval[c]) continue;
+    const int il = tx * TX + c / TY, xl = ty * TY + c % TY;
+    const int64_t idx = ((int64_t)il * a.n_xl + xl) * a.n_tau + k;
+    if constexpr (MODE == MODE_FWD) {
+      if (a.comp) {
+        float s = a.image[idx], cc = a.comp[idx];
+        neumaier(s, cc, acc[c]);
+        a.image[idx] = s;
+        a.comp[idx] = cc + cmp[c];
+      } else {
+        a.image[idx] += acc[c] + cmp[c];
+      }
+      if (a.fold) a.fold[idx] += (float)fcnt[c];
+    } else if constexpr (MODE == MODE_GRAD) {
+      a.vel_grad[idx] += (acc[c] + cmp[c]) * a.image_in[idx];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side dispatch
+
+template <int MODE, int TX, int TY>
+static int launch_shape(const pstm_params_t* p, const TileArgs& a, hipStream_t s) {
+  const int n_tiles = a.n_tiles_x * a.n_tiles_y;
+  const int chunk = (n_tiles + 7) / 8;
+  const int64_t blocks = (int64_t)8 * chunk * a.n_tblocks;
+  if (blocks <= 0) return PSTM_OK;
+  if (blocks > 0x7fffffff) return set_error(PSTM_ERR_INVALID, "too many tiles (%lld)", (long long)blocks);
+  const bool aa = (p->flags & PSTM_FLAG_AA) != 0, vti = (p->flags & PSTM_FLAG_VTI) != 0;
+  const int dt = p->dtype;
+  dim3 g((unsigned)blocks), b(kThreads);
+#define PSTM_LAUNCH(DTV, AAV, VTIV) \
+  hipLaunchKernelGGL((k_tile<MODE, TX, TY, DTV, AAV, VTIV>), g, b, 0, s, a)
+#define PSTM_DT(AAV, VTIV)                                   \
+  do {                                                       \
+    if (dt == PSTM_F32) PSTM_LAUNCH(PSTM_F32, AAV, VTIV);    \
+    else if (dt == PSTM_BF16) PSTM_LAUNCH(PSTM_BF16, AAV, VTIV); \
+    else PSTM_LAUNCH(PSTM_F16, AAV, VTIV);                   \
+  } while (0)
+  if constexpr (MODE == MODE_GRAD) {
+    if (dt == PSTM_F32) PSTM_LAUNCH(PSTM_F32, false, false);
+    else if (dt == PSTM_BF16) PSTM_LAUNCH(PSTM_BF16, false, false);
+    else PSTM_LAUNCH(PSTM_F16, false, false);
+  } else if constexpr (MODE == MODE_ADJ) {
+    if (aa && vti) PSTM_LAUNCH(PSTM_F32, true, true);
+    else if (aa) PSTM_LAUNCH(PSTM_F32, true, false);
+    else if (vti) PSTM_LAUNCH(PSTM_F32, false, true);
+    else PSTM_LAUNCH(PSTM_F32, false, false);
+  } else {
+    if (aa && vti) PSTM_DT(true, true);
+    else if (aa) PSTM_DT(true, false);
+    else if (vti) PSTM_DT(false, true);
+    else PSTM_DT(false, false);
+  }
+#undef PSTM_DT
+#undef PSTM_LAUNCH
+  return PSTM_OK;
+}
+
+template <int MODE>
+static int launch_mode(const pstm_params_t* p, const PlanLayout& L, const TileArgs& a, hipStream_t s) {
+  int rc;
+  if (L.tile_x == 4 && L.tile_y == 4) rc = launch_shape<MODE, 4, 4>(p, a, s);
+  else if (L.tile_x == 16 && L.tile_y == 1) rc = launch_shape<MODE, 16, 1>(p, a, s);
+  else if (L.tile_x == 1 && L.tile_y == 16) rc = launch_shape<MODE, 1, 16>(p, a, s);
+  else return set_error(PSTM_ERR_INVALID, "unsupported tile shape %dx%d", L.tile_x, L.tile_y);
+  if (rc != PSTM_OK) return rc;
+  return check_launch(MODE == MODE_FWD ? "forward" : MODE == MODE_ADJ ? "adjoint" : "vel_grad");
+}
+
+int launch_forward(const pstm_params_t* p, const PlanLayout& L, const TileArgs& a, hipStream_t s) {
+  return launch_mode<MODE_FWD>(p, L, a, s);
+}
+int launch_adjoint(const pstm_params_t* p, const PlanLayout& L, const TileArgs& a, hipStream_t s) {
+  return launch_mode<MODE_ADJ>(p, L, a, s);
+}
+int launch_vel_grad(const pstm_params_t* p, const PlanLayout& L, const TileArgs& a, hipStream_t s) {
+  return launch_mode<MODE_GRAD>(p, L, a, s);
+}
+
+}  // namespace pstm

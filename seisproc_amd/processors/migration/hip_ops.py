@@ -1,0 +1,207 @@
+"""Tensor-level wrappers of the HIP migration kernels (libpstm_hip.so).
+
+``PstmPlan`` owns the per-job device workspace (binned trace geometry) and
+exposes the three operators on torch tensors that live on the GPU:
+
+  forward   image[il][xl][tau] (+ fold)   <- traces      (Kirchhoff sum)
+  adjoint   traces                        <- image       (demigration)
+  vel_grad  d<g, image>/dv_rms per image sample (masks held fixed)
+
+All buffers are caller-owned torch tensors; the library allocates nothing.
+Work is enqueued on the current torch stream.  There is no CPU path: a
+missing GPU or library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Tuple
+
+import torch
+
+from ...abi_types import (FLAG_AA, FLAG_KAHAN, FLAG_VTI, PSTM_BF16, PSTM_F16, PSTM_F32, VEL_1D,
+                          VEL_3D, PstmParams)
+from ..kernel_backend import check, load_library
+
+_DTYPE_CODE = {torch.float32: PSTM_F32, torch.bfloat16: PSTM_BF16, torch.float16: PSTM_F16}
+
+
+def _stream_handle() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _need_cuda(t: torch.Tensor, name: str, dtype=None, shape=None) -> torch.Tensor:
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if t.device.type != "cuda":
+        raise ValueError(f"{name} must be on the GPU (got {t.device}); the migration path has no CPU fallback")
+    if dtype is not None and t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(t.shape)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return t
+
+
+class PstmPlan:
+    """Device plan of one migration job (geometry binned once, reused by every
+    forward / adjoint / gradient call of the job).
+
+    params:  PstmParams (or dict) describing traces, image grid and options.
+    src_xy, rcv_xy: (n_traces, 2) float64 world coordinates, GPU or host;
+             host arrays are copied once (FP64 is kept until the on-GPU
+             reduction to local coordinates).
+    """
+
+    def __init__(self, params, src_xy, rcv_xy, device: Optional[torch.device] = None):
+        self.lib = load_library()
+        p = params if isinstance(params, PstmParams) else PstmParams.from_dict(params)
+        self.params = p
+        if not torch.cuda.is_available():
+            raise RuntimeError("PstmPlan needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        src = torch.as_tensor(src_xy, dtype=torch.float64).to(self.device).contiguous()
+        rcv = torch.as_tensor(rcv_xy, dtype=torch.float64).to(self.device).contiguous()
+        n = int(p.n_traces)
+        _need_cuda(src, "src_xy", torch.float64, (n, 2))
+        _need_cuda(rcv, "rcv_xy", torch.float64, (n, 2))
+        check(self.lib.pstm_validate(ctypes.byref(p)), "pstm_validate", self.lib)
+        nbytes = int(self.lib.pstm_plan_bytes(ctypes.byref(p)))
+        if nbytes == 0:
+            check(-1, "pstm_plan_bytes", self.lib)
+        self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        check(self.lib.pstm_plan_build(ctypes.byref(p), _ptr(src), _ptr(rcv), _ptr(self.workspace),
+                                       nbytes, ctypes.c_void_p(_stream_handle())),
+              "pstm_plan_build", self.lib)
+        self._src, self._rcv = src, rcv  # kept alive until the build has run
+
+    # ------------------------------------------------------------------
+    @property
+    def image_shape(self) -> Tuple[int, int, int]:
+        p = self.params
+        return (p.n_il, p.n_xl, p.n_tau)
+
+    def kept(self) -> int:
+        """Number of traces whose midpoint can reach the image (synchronises)."""
+        return int(check(self.lib.pstm_plan_kept(_ptr(self.workspace), ctypes.c_void_p(_stream_handle())),
+                         "pstm_plan_kept", self.lib))
+
+    def _vel_mode(self, vel: torch.Tensor) -> int:
+        p = self.params
+        if vel.dim() == 1:
+            _need_cuda(vel, "vel", torch.float32, (p.n_tau,))
+            return VEL_1D
+        _need_cuda(vel, "vel", torch.float32, self.image_shape)
+        return VEL_3D
+
+    def _call_params(self, flags: Optional[int], dtype_code: int, vel_mode: int) -> PstmParams:
+        p = self.params
+        return p.replace(flags=p.flags if flags is None else flags, dtype=dtype_code, vel_mode=vel_mode)
+
+    # ------------------------------------------------------------------
+    def forward(self, traces: torch.Tensor, vel: torch.Tensor, *, scale: Optional[torch.Tensor] = None,
+                eta: Optional[torch.Tensor] = None, image: Optional[torch.Tensor] = None,
+                comp: Optional[torch.Tensor] = None, fold: Optional[torch.Tensor] = None,
+                with_fold: bool = True, flags: Optional[int] = None):
+        """Accumulate the migrated image (and fold) of ``traces``.
+
+        traces: (n_traces, n_samples) float32 / bfloat16 / float16 (trace-major)
+        scale:  (n_traces,) float32 per-trace scale (low-precision traces)
+        vel:    (n_tau,) or (n_il, n_xl, n_tau) float32 RMS velocity
+        eta:    same layout as vel (PSTM_FLAG_VTI)
+        image/comp/fold: optional accumulators (allocated as zeros if None).
+        Returns (image, fold) -- fold is None when with_fold is False.
+        """
+        p = self.params
+        if traces.dtype not in _DTYPE_CODE:
+            raise ValueError(f"unsupported trace dtype {traces.dtype}")
+        _need_cuda(traces, "traces", traces.dtype, (p.n_traces, p.n_samples))
+        vm = self._vel_mode(vel)
+        cp = self._call_params(flags, _DTYPE_CODE[traces.dtype], vm)
+        if scale is not None:
+            _need_cuda(scale, "scale", torch.float32, (p.n_traces,))
+        if cp.flags & FLAG_VTI:
+            if eta is None:
+                raise ValueError("VTI migration needs eta")
+            _need_cuda(eta, "eta", torch.float32, tuple(vel.shape))
+        if image is None:
+            image = torch.zeros(self.image_shape, dtype=torch.float32, device=self.device)
+        _need_cuda(image, "image", torch.float32, self.image_shape)
+        if comp is not None:
+            _need_cuda(comp, "comp", torch.float32, self.image_shape)
+        if fold is None and with_fold:
+            fold = torch.zeros(self.image_shape, dtype=torch.float32, device=self.device)
+        if fold is not None:
+            _need_cuda(fold, "fold", torch.float32, self.image_shape)
+        check(self.lib.pstm_forward(ctypes.byref(cp), _ptr(self.workspace), _ptr(traces), _ptr(scale),
+                                    _ptr(vel), _ptr(eta if cp.flags & FLAG_VTI else None), _ptr(image),
+                                    _ptr(comp), _ptr(fold), ctypes.c_void_p(_stream_handle())),
+              "pstm_forward", self.lib)
+        return image, fold
+
+    def adjoint(self, image: torch.Tensor, vel: torch.Tensor, *, eta: Optional[torch.Tensor] = None,
+                out: Optional[torch.Tensor] = None, flags: Optional[int] = None) -> torch.Tensor:
+        """Demigrate ``image`` into FP32 traces (accumulates into ``out``)."""
+        p = self.params
+        _need_cuda(image, "image", torch.float32, self.image_shape)
+        vm = self._vel_mode(vel)
+        cp = self._call_params(flags, PSTM_F32, vm)
+        if cp.flags & FLAG_VTI:
+            if eta is None:
+                raise ValueError("VTI migration needs eta")
+            _need_cuda(eta, "eta", torch.float32, tuple(vel.shape))
+        if out is None:
+            out = torch.zeros((p.n_traces, p.n_samples), dtype=torch.float32, device=self.device)
+        _need_cuda(out, "out", torch.float32, (p.n_traces, p.n_samples))
+        check(self.lib.pstm_adjoint(ctypes.byref(cp), _ptr(self.workspace), _ptr(image), _ptr(vel),
+                                    _ptr(eta if cp.flags & FLAG_VTI else None), _ptr(out),
+                                    ctypes.c_void_p(_stream_handle())),
+              "pstm_adjoint", self.lib)
+        return out
+
+    def vel_grad(self, traces: torch.Tensor, vel: torch.Tensor, image_grad: torch.Tensor, *,
+                 scale: Optional[torch.Tensor] = None, out: Optional[torch.Tensor] = None,
+                 flags: Optional[int] = None) -> torch.Tensor:
+        """d<image_grad, forward(traces; vel)>/d vel per image sample."""
+        p = self.params
+        if traces.dtype not in _DTYPE_CODE:
+            raise ValueError(f"unsupported trace dtype {traces.dtype}")
+        _need_cuda(traces, "traces", traces.dtype, (p.n_traces, p.n_samples))
+        _need_cuda(vel, "vel", torch.float32, self.image_shape)
+        _need_cuda(image_grad, "image_grad", torch.float32, self.image_shape)
+        if scale is not None:
+            _need_cuda(scale, "scale", torch.float32, (p.n_traces,))
+        cp = self._call_params(flags, _DTYPE_CODE[traces.dtype], VEL_3D)
+        if out is None:
+            out = torch.zeros(self.image_shape, dtype=torch.float32, device=self.device)
+        _need_cuda(out, "out", torch.float32, self.image_shape)
+        check(self.lib.pstm_vel_grad(ctypes.byref(cp), _ptr(self.workspace), _ptr(traces), _ptr(scale),
+                                     _ptr(vel), _ptr(image_grad), _ptr(out),
+                                     ctypes.c_void_p(_stream_handle())),
+              "pstm_vel_grad", self.lib)
+        return out
+
+
+def quantize_traces(traces: torch.Tensor, dtype: torch.dtype) -> Tuple[torch.Tensor, torch.Tensor]:
+    """FP32 traces -> (bf16/fp16 traces, per-trace FP32 scale) on the GPU
+    (scale = max|trace|, round to nearest even)."""
+    lib = load_library()
+    if dtype not in (torch.bfloat16, torch.float16):
+        raise ValueError("dtype must be torch.bfloat16 or torch.float16")
+    _need_cuda(traces, "traces", torch.float32)
+    if traces.dim() != 2:
+        raise ValueError("traces must be (n_traces, n_samples)")
+    n, ns = traces.shape
+    out = torch.empty((n, ns), dtype=dtype, device=traces.device)
+    scale = torch.empty((n,), dtype=torch.float32, device=traces.device)
+    check(lib.pstm_quantize_traces(_ptr(traces), n, ns, _DTYPE_CODE[dtype], _ptr(out), _ptr(scale),
+                                   ctypes.c_void_p(_stream_handle())),
+          "pstm_quantize_traces", lib)
+    return out, scale
+
+
+__all__ = ["PstmPlan", "quantize_traces", "FLAG_AA", "FLAG_KAHAN", "FLAG_VTI"]

@@ -1,0 +1,300 @@
+"""Pins of the FP64 oracle (oracle/pstm_oracle.c).
+
+Known answers are restated from the reference's own tests:
+  tests/test_geometry_preprocessor.py:123-218  DSR traveltimes
+  tests/test_geometry_preprocessor.py:224-316  weights and masks
+  tests/test_kirchhoff_kernel.py:33-105        clamped linear interpolation
+  tests/test_kirchhoff_kernel.py:272-328       diffractor focusing
+plus closed forms, brute force and the invariants that fix the rest.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import numpy_ref
+from seisproc_amd import synthetic as S
+from seisproc_amd.abi_types import (FLAG_AA, FLAG_TAPER, FLAG_TIME_APERTURE, FLAG_VTI,
+                                    FLAG_ZERO_OUTSIDE, WEIGHT_NONE, WEIGHT_SPREADING, PstmParams)
+
+
+def _p(**kw):
+    base = dict(n_traces=1, n_samples=2001, n_il=1, n_xl=1, n_tau=1, dt=0.002, t0=0.0, dtau=0.002,
+                tau0=0.0, aperture_m=5000.0, max_angle_deg=60.0)
+    base.update(kw)
+    return PstmParams.from_dict(base)
+
+
+# ---------------------------------------------------------------- traveltime
+@pytest.mark.parametrize("h,z,expect_ms", [
+    (0.0, 1500.0, 1000.0),                                        # test_geometry_preprocessor.py:123-144
+    (1000.0, 1500.0, 2 * math.sqrt(1000**2 + 1500**2) / 3000 * 1000),  # :146-168
+    (0.0, 500.0, 2 * 500 / 3000 * 1000), (0.0, 2000.0, 2 * 2000 / 3000 * 1000),  # :170-192
+])
+def test_dsr_time_known_answers(oracle_lib, h, z, expect_ms):
+    v = 3000.0
+    tau = 2 * z / v
+    ok, w, u, L = oracle.pair(_p(), h, 0.0, h, 0.0, tau, v)
+    assert ok
+    assert abs(u * 2.0 - expect_ms) < 0.01  # u samples of 2 ms
+
+
+def test_dsr_time_asymmetric_legs(oracle_lib):
+    # source and receiver on opposite sides: t = (r_s + r_r)/v
+    v, z = 2500.0, 1200.0
+    tau = 2 * z / v
+    ok, w, u, L = oracle.pair(_p(), 300.0, 400.0, -800.0, 0.0, tau, v)
+    t = (math.hypot(500.0, z) + math.hypot(800.0, z)) / v
+    assert ok and abs(u * 0.002 - t) < 1e-12
+
+
+# ---------------------------------------------------------------- weights / masks
+def test_weight_maximal_at_zero_offset_and_decreasing(oracle_lib):
+    # test_geometry_preprocessor.py:224-264
+    tau = 2 * 1500.0 / 3000.0
+    ws = [oracle.pair(_p(), d, 0, d, 0, tau, 3000.0)[1] for d in (0.0, 500.0, 1000.0, 1500.0)]
+    assert ws[0] > 0 and all(a >= b for a, b in zip(ws, ws[1:]))
+    # closed form at zero offset: (z/(r+e))^2 / (r^2 + e) with r = z
+    z = 1500.0
+    assert abs(ws[0] - (z / (z + 1e-6)) ** 2 / (z * z + 1e-6)) < 1e-18
+
+
+def test_aperture_mask(oracle_lib):
+    # test_geometry_preprocessor.py:266-290: z = 3000 m, A = 5000 m, 60 deg
+    tau = 2 * 3000.0 / 3000.0
+    res = [oracle.pair(_p(), d, 0, d, 0, tau, 3000.0)[0] for d in (0.0, 2000.0, 6000.0)]
+    assert res == [True, True, False]
+
+
+def test_angle_mask(oracle_lib):
+    # test_geometry_preprocessor.py:292-316: z = 100 m, 45 deg
+    tau = 2 * 100.0 / 3000.0
+    p = _p(max_angle_deg=45.0)
+    res = [oracle.pair(p, d, 0, d, 0, tau, 3000.0)[0] for d in (0.0, 500.0, 1000.0)]
+    assert res == [True, False, False]
+    # exactly at the limit: atan2(h, z + 1e-6) < 45 deg is strict
+    assert not oracle.pair(p, 100.0 + 1e-6, 0, 0, 0, tau, 3000.0)[0]
+    assert oracle.pair(p, 99.99, 0, 0, 0, tau, 3000.0)[0]
+
+
+def test_time_dependent_aperture(oracle_lib):
+    # kirchhoff_kernel.py:673-678: aperture(z) = min(A, z tan(theta))
+    p = _p(flags=FLAG_TIME_APERTURE, aperture_m=5000.0, max_angle_deg=45.0)
+    tau = 2 * 1000.0 / 2000.0  # z = 1000
+    assert oracle.pair(p, 999.0, 0, 0, 0, tau, 2000.0)[0]
+    assert not oracle.pair(p, 1000.5, 0, 0, 0, tau, 2000.0)[0]
+
+
+def test_spreading_weight(oracle_lib):
+    # kirchhoff_migrator.py:378: 1/(sqrt(r_s^2 r_r^2) + 1e-6)
+    p = _p(weight_mode=WEIGHT_SPREADING)
+    z, v = 1000.0, 2000.0
+    ok, w, u, L = oracle.pair(p, 300.0, 0, -200.0, 0, 2 * z / v, v)
+    rs, rr = math.hypot(300, z), math.hypot(200, z)
+    assert abs(w - 1.0 / (rs * rr + 1e-6)) < 1e-18
+
+
+# ---------------------------------------------------------------- interpolation
+def _column_trace(samples, tau_list, t0=0.0, dt=0.002, weight=WEIGHT_NONE, flags=0, aa_max=16.0):
+    """Zero-offset trace at the single image column: t = tau exactly."""
+    ns = len(samples)
+    nt = len(tau_list)
+    dtau = tau_list[1] - tau_list[0] if nt > 1 else 1.0
+    p = _p(n_samples=ns, n_tau=nt, t0=t0, dt=dt, dtau=dtau, tau0=tau_list[0], weight_mode=weight,
+           flags=flags, aa_max=aa_max, vel_mode=0)
+    v = np.full(nt, 2000.0)
+    img, fold = oracle.forward(p, np.asarray(samples, float)[None, :], [0.0], [0.0], [0.0], [0.0], v)
+    return img[0, 0], fold[0, 0]
+
+
+def test_interpolation_integer_and_half_sample(oracle_lib):
+    # test_kirchhoff_kernel.py:33-57
+    img, _ = _column_trace([0, 1, 2, 3, 4], [0.004, 0.005])
+    assert abs(img[0] - 2.0) < 1e-12
+    img, _ = _column_trace([0, 2, 4, 6, 8], [0.003, 0.004])
+    assert abs(img[0] - 3.0) < 1e-12
+
+
+def test_interpolation_clamps(oracle_lib):
+    # test_kirchhoff_kernel.py:59-86: before the record -> sample 0; after -> ~last
+    img, fold = _column_trace([1, 2, 3, 4, 5], [0.010, 0.100], t0=0.02)
+    assert abs(img[0] - 1.0) < 1e-3 and abs(img[1] - 5.0) < 1e-3
+    assert list(fold) == [1.0, 1.0]  # clamped contributions still count
+    img, fold = _column_trace([1, 2, 3, 4, 5], [0.010, 0.100], t0=0.02, flags=FLAG_ZERO_OUTSIDE)
+    assert list(fold) == [0.0, 0.0] and list(img) == [0.0, 0.0]
+
+
+def test_interpolation_matches_np_interp(oracle_lib):
+    rng = np.random.default_rng(0)
+    d = rng.normal(size=200)
+    taus = list(0.0123 + 0.00137 * np.arange(150))
+    img, _ = _column_trace(d, taus)
+    ref = np.interp(np.asarray(taus) / 0.002, np.arange(200), d)
+    np.testing.assert_allclose(img, ref, rtol=0, atol=1e-12)
+
+
+# ---------------------------------------------------------------- antialias
+def test_triangle_L1_equals_linear_interpolation(oracle_lib):
+    rng = np.random.default_rng(1)
+    d = rng.normal(size=300)
+    taus = list(0.0031 + 0.00171 * np.arange(160))
+    lin, _ = _column_trace(d, taus)
+    tri, _ = _column_trace(d, taus, flags=FLAG_AA, aa_max=1.0)
+    np.testing.assert_allclose(tri, lin, rtol=0, atol=1e-8)
+
+
+def test_triangle_brute_force_convolution(oracle_lib):
+    # Image column and trace offset so the operator dip exceeds aa_max: L
+    # clamps to exactly aa_max = 3 and the taps equal a 5-point triangle
+    # (weights (3-|j|)/9) of the zero-extended trace, interpolated linearly.
+    rng = np.random.default_rng(2)
+    ns, L = 400, 3
+    d = rng.normal(size=ns)
+    p = _p(n_samples=ns, n_tau=120, dtau=0.0031, tau0=0.05, flags=FLAG_AA, aa_max=float(L),
+           aa_scale=50.0, weight_mode=WEIGHT_NONE, aperture_m=1e5, max_angle_deg=89.0)
+    v = np.full(120, 2000.0)
+    sx = [300.0]
+    img, fold = oracle.forward(p, d[None, :], sx, [0.0], sx, [0.0], v)
+    ext = np.concatenate([np.zeros(8), d, np.zeros(8)])
+    tri = np.array([sum((L - abs(j)) / L**2 * ext[i + 8 + j] for j in range(-L + 1, L)) for i in range(ns)])
+    tri = np.append(tri, 0.0)
+    checked = 0
+    for k in range(120):
+        tau = 0.05 + 0.0031 * k
+        ok, w, u, Lk = oracle.pair(p, -300.0, 0, -300.0, 0, tau, 2000.0)
+        if not ok:
+            assert fold[0, 0, k] == 0
+            continue
+        assert Lk == L
+        i = int(u)
+        f = u - i
+        assert abs(img[0, 0, k] - ((1 - f) * tri[i] + f * tri[i + 1])) < 1e-8
+        checked += 1
+    assert checked > 100
+
+
+# ---------------------------------------------------------------- VTI
+def test_vti_closed_form_and_isotropic_limit(oracle_lib):
+    # Alkhalifah anelliptic: h = 1000, z = 1000, v = 2000, eta = 0.1
+    # t0 = 0.5, x^2/v^2 = 0.25, den = 4e6 (1e6 + 1.2e6) = 8.8e12,
+    # corr = 0.2e12 / 8.8e12 -> t^2 = 0.5 - 0.0227272727...
+    t = oracle.leg_time_vti(1000.0**2, 1000.0, 2000.0, 0.1)
+    assert abs(t - math.sqrt(0.5 - 0.2 / 8.8)) < 1e-14
+    # eta = 0 equals the straight ray r / v (traveltime_vti.py isotropic limit)
+    assert abs(oracle.leg_time_vti(700.0**2, 1300.0, 2400.0, 0.0) - math.hypot(700, 1300) / 2400) < 1e-14
+    # eta > 0 shortens far-offset times
+    assert oracle.leg_time_vti(3000.0**2, 1000.0, 2000.0, 0.15) < math.hypot(3000, 1000) / 2000
+
+
+def test_vti_forward_equals_isotropic_at_eta_zero(oracle_lib):
+    s = S.tiny_survey(seed=3, n_bins=6, n_classes=2, n_samples=151, n_img=4, aperture=300.0)
+    sx, sy, rx, ry = s.numpy_geometry()
+    p = s.params.replace(vel_mode=0)
+    v = s.vel.numpy()
+    d = s.traces.numpy()
+    iso, fi = oracle.forward(p, d, sx, sy, rx, ry, v)
+    vti, fv = oracle.forward(p.replace(flags=FLAG_VTI), d, sx, sy, rx, ry, v, eta=np.zeros_like(v))
+    np.testing.assert_array_equal(fi, fv)
+    np.testing.assert_allclose(vti, iso, rtol=1e-9, atol=1e-9 * np.abs(iso).max())
+
+
+# ---------------------------------------------------------------- brute force
+@pytest.mark.parametrize("flags,lateral,wm", [
+    (0, False, 0), (FLAG_AA, False, 0), (FLAG_AA | FLAG_VTI, True, 0), (FLAG_TAPER, True, 1),
+    (FLAG_TIME_APERTURE | FLAG_ZERO_OUTSIDE, False, 2),
+])
+def test_oracle_matches_numpy_brute_force(oracle_lib, flags, lateral, wm):
+    s = S.tiny_survey(seed=4, n_bins=5, n_classes=2, n_samples=121, n_img=3, aperture=250.0, flags=flags,
+                      lateral=lateral)
+    sx, sy, rx, ry = s.numpy_geometry()
+    p = s.params.replace(weight_mode=wm, vel_mode=1 if s.vel.dim() == 3 else 0)
+    d = s.traces.double().numpy()
+    v = s.vel.double().numpy()
+    eta = s.eta.double().numpy() if s.eta is not None else None
+    img, fold = oracle.forward(p, d, sx, sy, rx, ry, v, eta=eta)
+    oracle.set_exhaustive(True)
+    try:
+        img_x, fold_x = oracle.forward(p, d, sx, sy, rx, ry, v, eta=eta)
+    finally:
+        oracle.set_exhaustive(False)
+    ref, rfold = numpy_ref.forward(p.to_dict(), d, sx, sy, rx, ry, v, eta)
+    assert fold.sum() > 0
+    np.testing.assert_array_equal(fold, rfold)
+    np.testing.assert_array_equal(fold_x, rfold)
+    np.testing.assert_allclose(img, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+    np.testing.assert_allclose(img_x, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+
+
+# ---------------------------------------------------------------- adjoint
+@pytest.mark.parametrize("flags", [0, FLAG_AA, FLAG_AA | FLAG_VTI | FLAG_TAPER])
+def test_oracle_dot_product(oracle_lib, flags):
+    s = S.tiny_survey(seed=5, n_bins=6, n_classes=2, n_samples=101, n_img=4, aperture=300.0,
+                      flags=flags & (FLAG_AA | FLAG_VTI), lateral=True)
+    sx, sy, rx, ry = s.numpy_geometry()
+    p = s.params.replace(flags=flags, vel_mode=1)
+    v = s.vel.double().numpy()
+    if v.ndim == 1:
+        v = np.broadcast_to(v, (p.n_il, p.n_xl, p.n_tau)).copy()
+    eta = s.eta.double().numpy() if s.eta is not None else None
+    rng = np.random.default_rng(7)
+    m = rng.normal(size=(p.n_il, p.n_xl, p.n_tau))
+    d = rng.normal(size=(p.n_traces, p.n_samples))
+    Lm, _ = oracle.forward(p, d * 0 + 0, sx, sy, rx, ry, v, eta=eta)  # zero check
+    assert np.all(Lm == 0)
+    Ld, _ = oracle.forward(p, d, sx, sy, rx, ry, v, eta=eta)
+    Ltm = oracle.adjoint(p, m, sx, sy, rx, ry, v, eta=eta)
+    lhs, rhs = float(np.sum(Ld * m)), float(np.sum(d * Ltm))
+    assert abs(lhs) > 0
+    assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), abs(rhs))
+
+
+# ---------------------------------------------------------------- focusing
+def test_diffractor_focuses_at_tau0(oracle_lib):
+    # Reduced form of tests/fixtures/synthetic_diffractor.py:102-228 (zero
+    # offset, point diffractor, v = 3000): the migrated peak sits at the
+    # diffractor's tau0 and column (test_kirchhoff_kernel.py:272-328), not
+    # below it (the sign error of the time-domain kernels).
+    n, d, v = 21, 50.0, 3000.0
+    xd, yd, zd = 500.0, 500.0, 600.0
+    ix = np.arange(n) * d
+    mx, my = np.meshgrid(ix, ix, indexing="ij")
+    mx, my = mx.ravel(), my.ravel()
+    dt, ns = 0.002, 501
+    r = np.sqrt((mx - xd) ** 2 + (my - yd) ** 2 + zd**2)
+    tt = np.arange(ns) * dt
+    sig = 1.0 / (math.pi * 30.0 * math.sqrt(2))
+    tr = np.array([(1 - ((tt - 2 * ri / v) / sig) ** 2) * np.exp(-0.5 * ((tt - 2 * ri / v) / sig) ** 2) / ri
+                   for ri in r])
+    p = PstmParams.from_dict(dict(n_traces=n * n, n_samples=ns, n_il=n, n_xl=n, n_tau=ns, dt=dt, dtau=dt,
+                                  il_dx=d, il_dy=0.0, xl_dx=0.0, xl_dy=d, aperture_m=5000.0,
+                                  max_angle_deg=60.0))
+    img, fold = oracle.forward(p, tr, mx, my, mx, my, np.full(ns, v))
+    norm = np.where(fold >= 1, img / np.maximum(fold, 1), img)
+    il, xl, k = np.unravel_index(np.argmax(np.abs(norm)), norm.shape)
+    assert abs(k - round(2 * zd / v / dt)) <= 3
+    assert abs(il - 10) <= 1 and abs(xl - 10) <= 1
+
+
+# ---------------------------------------------------------------- velocity gradient
+@pytest.mark.parametrize("wm", [0, 1])
+def test_oracle_vel_grad_central_differences(oracle_lib, wm):
+    s = S.tiny_survey(seed=6, n_bins=6, n_classes=2, n_samples=121, n_img=3, aperture=300.0, lateral=True)
+    sx, sy, rx, ry = s.numpy_geometry()
+    p = s.params.replace(vel_mode=1, weight_mode=wm)
+    v0 = s.vel.double().numpy()
+    d = s.traces.double().numpy()
+    rng = np.random.default_rng(11)
+    g = rng.normal(size=v0.shape)
+    grad = oracle.vel_grad(p, d, sx, sy, rx, ry, v0, g)
+    assert np.abs(grad).max() > 0
+    # J(v) = <g, L(v) d> with the masks frozen at v0; perturb along a smooth
+    # direction, central differences with a small step (1e-3 m/s) so that no
+    # contribution crosses a linear-interpolation knot.
+    delta = np.cos(np.linspace(0, 3, v0.size)).reshape(v0.shape)
+    h = 1e-3
+    Jp = np.sum(g * oracle.forward(p, d, sx, sy, rx, ry, v0 + h * delta, mask_vel=v0)[0])
+    Jm = np.sum(g * oracle.forward(p, d, sx, sy, rx, ry, v0 - h * delta, mask_vel=v0)[0])
+    fd = (Jp - Jm) / (2 * h)
+    an = float(np.sum(grad * delta))
+    assert abs(fd - an) <= 1e-5 * abs(an)
