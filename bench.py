@@ -1,0 +1,269 @@
+"""PSTM throughput benchmark on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg3|cfg2|cfg5]
+
+Default workload (the metric's precision form): 3D PSTM of 25 offset classes
+x 40,000 traces (1,000,000 traces x 1001 samples at 2 ms) into a 400 x 400 x
+1001 image at 12.5 m, aperture 1500 m / 60 deg, compensated accumulation,
+traces stored BF16 with per-trace FP32 scales.  A step is one forward
+migration of every trace into the full image (fold included) with the inputs
+resident in HBM.  ``value`` is the BF16-trace rate in Gcontrib/s (one
+contribution = one (trace, image sample) pair that passes the aperture and
+angle tests = one unit of fold); the FP32 and FP16 forms are timed beside it
+and every form's error against the FP64 oracle is measured on sampled image
+columns.
+
+The CPU baseline is the torch-CPU restatement of the reference's CPU method
+(oracle/ref_torch.py), timed on a bounded sample of the same workload: one
+100-point reference tile (10 x 10 image columns) at every 8th output time,
+with every trace that reaches it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg3", choices=["cfg3", "cfg2", "cfg5"])
+    ap.add_argument("--classes", type=int, default=None, help="offset classes (cfg3: 25)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--cpu-tau-stride", type=int, default=8)
+    return ap.parse_args()
+
+
+def _timed(fn, steps, torch):
+    """Time `steps` calls of fn on the current stream: HIP events bracket the
+    launches (kernel-time per launch) and synchronize() brackets the wall."""
+    s = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(s)
+    for _ in range(steps):
+        fn()
+    e1.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall / steps, e0.elapsed_time(e1) / 1e3 / steps
+
+
+def _workload(args, torch):
+    from seisproc_amd import synthetic as S
+    dev = "cuda"
+    if args.workload == "cfg3":
+        ncls = args.classes or 25
+        s = S.config3_precision(n_classes=ncls, device=dev)
+        cfg = {"workload": "pstm3d-precision", "offset_classes": ncls, "traces": s.n_traces,
+               "samples": s.params.n_samples, "dt_ms": 2.0,
+               "image": [s.params.n_il, s.params.n_xl, s.params.n_tau], "bin_m": 12.5,
+               "aperture_m": 1500.0, "max_angle_deg": 60.0, "antialias": False, "kahan": True,
+               "velocity": "v_rms = 2000 m/s"}
+        forms = ["bf16", "fp32", "fp16"]
+    elif args.workload == "cfg2":
+        s = S.config2_3d(device=dev)
+        cfg = {"workload": "pstm3d-single-offset", "traces": s.n_traces, "samples": s.params.n_samples,
+               "image": [s.params.n_il, s.params.n_xl, s.params.n_tau], "bin_m": 25.0,
+               "aperture_m": 3000.0, "antialias": True,
+               "velocity": "v_rms = (1500 + 600 t)(1 +- 5% lateral)"}
+        forms = ["fp32", "bf16"]
+    else:
+        s = S.config5_vti(device=dev)
+        cfg = {"workload": "pstm3d-vti-largest", "traces": s.n_traces, "classes": len(s.class_sizes),
+               "samples": s.params.n_samples, "image": [s.params.n_il, s.params.n_xl, s.params.n_tau],
+               "bin_m": 12.5, "aperture_m": 4000.0, "antialias": True, "vti": True}
+        forms = ["fp16", "fp32"]
+    return s, cfg, forms
+
+
+def _error_columns(s, plan, images, torch):
+    """FP64 oracle on a few sampled image columns vs every form's image."""
+    import numpy as np
+    import oracle
+    p = s.params
+    picks = [(p.n_il // 2, p.n_xl // 2), (p.n_il // 4, (3 * p.n_xl) // 4), ((3 * p.n_il) // 4, p.n_xl // 3)]
+    sx, sy, rx, ry = s.numpy_geometry()
+    tr = s.traces
+    vel_full = s.vel
+    refs, gots = [], {k: [] for k in images}
+    for il, xl in picks:
+        x = p.origin_x + il * p.il_dx + xl * p.xl_dx
+        y = p.origin_y + il * p.il_dy + xl * p.xl_dy
+        mx, my = 0.5 * (sx + rx), 0.5 * (sy + ry)
+        sel = np.nonzero((mx - x) ** 2 + (my - y) ** 2 < p.aperture_m ** 2)[0]
+        q = p.replace(n_traces=len(sel), n_il=1, n_xl=1, origin_x=x, origin_y=y, vel_mode=0)
+        vcol = vel_full if vel_full.dim() == 1 else vel_full[il, xl]
+        eta = None
+        if s.eta is not None:
+            eta = s.eta[il, xl].double().cpu().numpy()
+        d = tr[torch.as_tensor(sel, device=tr.device)].double().cpu().numpy()
+        ref, _ = oracle.forward(q, d, sx[sel], sy[sel], rx[sel], ry[sel], vcol.double().cpu().numpy(), eta=eta)
+        refs.append(ref.ravel())
+        for k, img in images.items():
+            gots[k].append(img[il, xl].double().cpu().numpy())
+    ref = np.concatenate(refs)
+    out = {}
+    for k in images:
+        g = np.concatenate(gots[k])
+        out[k] = {"rel_rms": float(np.linalg.norm(g - ref) / np.linalg.norm(ref)),
+                  "rel_max": float(np.abs(g - ref).max() / np.abs(ref).max())}
+    return out, f"FP64 oracle on image columns {picks} (all traces in aperture)"
+
+
+def _cpu_baseline(s, stride, torch):
+    """Reference CPU method on one 100-point tile (10 x 10 columns) at every
+    `stride`-th output time, with every trace its pre-filter keeps."""
+    from oracle.ref_torch import migrate_kirchhoff_full_cpu
+    p = s.params
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    torch.set_num_threads(threads)
+    v = float(s.vel.flatten()[0]) if s.vel.dim() == 1 else None
+    if v is None or float(s.vel.max()) != float(s.vel.min()):
+        return None  # the reference CPU method has a constant velocity only
+    il0, xl0 = p.n_il // 2 - 5, p.n_xl // 2 - 5
+    il = torch.arange(il0, il0 + 10, dtype=torch.float64)
+    xl = torch.arange(xl0, xl0 + 10, dtype=torch.float64)
+    ox = (p.origin_x + il[:, None] * p.il_dx + xl[None, :] * p.xl_dx).reshape(-1).float()
+    oy = (p.origin_y + il[:, None] * p.il_dy + xl[None, :] * p.xl_dy).reshape(-1).float()
+    src, rcv = s.src_xy, s.rcv_xy
+    mx, my = 0.5 * (src[:, 0] + rcv[:, 0]), 0.5 * (src[:, 1] + rcv[:, 1])
+    cx, cy = float(ox.double().mean()), float(oy.double().mean())
+    reach = p.aperture_m + 10 * max(p.il_dx, p.xl_dy) + 100.0
+    sel = torch.nonzero((mx - cx) ** 2 + (my - cy) ** 2 < reach ** 2).flatten()
+    tr = s.traces[sel].t().contiguous().cpu()
+    g = lambda t: t[sel].float().cpu()
+    sx, sy, rx, ry = g(src[:, 0]), g(src[:, 1]), g(rcv[:, 0]), g(rcv[:, 1])
+    tau_ms = (torch.arange(0, p.n_tau, stride, dtype=torch.float32) * p.dtau * 1000.0)
+    depth = v * (tau_ms / 1000.0) / 2.0
+    t0 = time.perf_counter()
+    img, fold = migrate_kirchhoff_full_cpu(tr, sx, sy, rx, ry, ox, oy, depth, v, p.dt * 1000.0, 0.0,
+                                           p.aperture_m, p.max_angle_deg, 10, 10)
+    sec = time.perf_counter() - t0
+    contrib = float(fold.double().sum())
+    return {"value": contrib / sec / 1e9, "unit": "Gcontrib/s", "cores": threads, "kind": "port",
+            "sample": f"reference tile loop (torch CPU FP32, oracle/ref_torch.py) on 10x10 image columns "
+                      f"x every {stride}th of {p.n_tau} output times, {len(sel)} traces in reach; "
+                      f"{contrib:.3e} contributions in {sec:.1f} s"}
+
+
+def main():
+    args = _parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is design-only", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from seisproc_amd.processors.migration.hip_ops import PstmPlan, quantize_traces
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        sys.exit(1)
+    t_start = time.perf_counter()
+    s, cfg, forms = _workload(args, torch)
+    plan = PstmPlan(s.params, s.src_xy, s.rcv_xy)
+    vel = s.vel.contiguous()
+    eta = s.eta
+    tr32 = s.traces
+    data = {"fp32": (tr32, None)}
+    for f, dt in (("bf16", torch.bfloat16), ("fp16", torch.float16)):
+        if f in forms:
+            data[f] = quantize_traces(tr32, dt)
+    torch.cuda.synchronize()
+    print(f"[bench] {cfg['workload']}: {s.n_traces} traces, kept {plan.kept()}, setup "
+          f"{time.perf_counter() - t_start:.1f} s", file=sys.stderr, flush=True)
+
+    image = torch.zeros(plan.image_shape, dtype=torch.float32, device="cuda")
+    comp = torch.zeros_like(image)
+    fold = torch.zeros_like(image)
+
+    def step(form):
+        tr, sc = data[form]
+        image.zero_()
+        comp.zero_()
+        fold.zero_()
+        plan.forward(tr, vel, scale=sc, eta=eta, image=image, comp=comp, fold=fold)
+
+    main_form = forms[0]
+    results, images = {}, {}
+    for form in forms:
+        w = args.warmup if form == main_form else 1
+        k = args.steps if form == main_form else 1
+        for _ in range(w):
+            step(form)
+        torch.cuda.synchronize()
+        contrib = float(fold.double().sum())
+        wall, kern = _timed(lambda: step(form), k, torch)
+        # the zero_() fills are outside the kernel timing: time the kernel alone
+        tr, sc = data[form]
+        _, kern_only = _timed(lambda: plan.forward(tr, vel, scale=sc, eta=eta, image=image, comp=comp,
+                                                   fold=fold), 1, torch)
+        step(form)
+        torch.cuda.synchronize()
+        images[form] = (image + comp).cpu()
+        results[form] = {"wall_s": wall, "kernel_s": kern_only, "contributions": contrib,
+                         "gcontrib_s": contrib / wall / 1e9}
+        print(f"[bench] {form}: {wall * 1e3:.1f} ms/step, kernel {kern_only * 1e3:.1f} ms, "
+              f"{contrib / wall / 1e9:.1f} Gcontrib/s", file=sys.stderr, flush=True)
+
+    err, err_desc = ({}, None)
+    if not args.no_error:
+        err, err_desc = _error_columns(s, plan, images, torch)
+        print(f"[bench] errors vs FP64 oracle: {err}", file=sys.stderr, flush=True)
+
+    m = results[main_form]
+    aa = bool(s.params.flags & 1)
+    taps = 6 if aa else 2
+    sbytes = 4 if main_form == "fp32" else 2
+    B = taps * sbytes
+    achieved = m["contributions"] * B / m["kernel_s"] / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = _cpu_baseline(s, args.cpu_tau_stride, torch)
+    line = {
+        "metric": "PSTM Gcontrib/s (trace×image pt), BF16-trace vs FP32 at 1e-3 rel err",
+        "value": round(m["gcontrib_s"], 3),
+        "unit": "Gcontrib/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(m["wall_s"] * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"bf16": "bf16-traces/f32-compute", "fp16": "f16-traces/f32-compute",
+                  "fp32": "f32"}[main_form],
+        "data": "synthetic (seeded Ricker events at exact DSR times + noise)",
+        "config": cfg,
+        "contributions": m["contributions"],
+        "forms": {k: {"gcontrib_s": round(v["gcontrib_s"], 3), "ms_per_step": round(v["wall_s"] * 1e3, 3),
+                      "kernel_ms": round(v["kernel_s"] * 1e3, 3), **err.get(k, {})}
+                  for k, v in results.items()},
+        "error_check": err_desc,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_contribution": B},
+        "cpu_baseline": cpu,
+    }
+    if "fp32" in results and main_form != "fp32":
+        line["speedup_vs_fp32_form"] = round(m["gcontrib_s"] / results["fp32"]["gcontrib_s"], 3)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

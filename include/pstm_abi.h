@@ -53,9 +53,11 @@ enum {
   PSTM_FLAG_TIME_APERTURE = 1u << 5  /* aperture(z) = min(aperture_m, z*tan(max_angle)) */
 };
 
-/* Amplitude weight (reference: kirchhoff_kernel.py:691-693 is FULL,
- * kirchhoff_migrator.py:378 is SPREADING). */
-enum { PSTM_WEIGHT_FULL = 0, PSTM_WEIGHT_SPREADING = 1, PSTM_WEIGHT_NONE = 2 };
+/* Amplitude weight (reference: kirchhoff_kernel.py:691-693 is FULL =
+ * cos_s cos_r / (r_s r_r); kirchhoff_migrator.py:378 is SPREADING =
+ * 1 / (r_s r_r); OBLIQUITY = cos_s cos_r; NONE = 1;
+ * models/migration_config.py:40-45 names the four modes). */
+enum { PSTM_WEIGHT_FULL = 0, PSTM_WEIGHT_SPREADING = 1, PSTM_WEIGHT_NONE = 2, PSTM_WEIGHT_OBLIQUITY = 3 };
 
 /* Velocity / eta layout. 1D: one value per output tau, shared by all columns.
  * 3D: one value per image sample, [il][xl][tau]. */

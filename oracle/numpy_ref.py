@@ -62,8 +62,9 @@ def forward(p: dict, traces, sx, sy, rx, ry, vel, eta=None):
                     else:
                         t = (rs + rr) / v
                     wm = p.get("weight_mode", 0)
-                    w = (z / (rs + EPS)) * (z / (rr + EPS)) / (rs * rr + EPS) if wm == 0 else (
-                        1.0 / (rs * rr + EPS) if wm == 1 else 1.0)
+                    w = {0: (z / (rs + EPS)) * (z / (rr + EPS)) / (rs * rr + EPS),
+                         1: 1.0 / (rs * rr + EPS), 2: 1.0,
+                         3: (z / (rs + EPS)) * (z / (rr + EPS))}[wm]
                     if taper:
                         lim = min(ap, np.tan(th) * (z + EPS))
                         rho = max(hs, hr) / lim
@@ -75,8 +76,10 @@ def forward(p: dict, traces, sx, sy, rx, ry, vel, eta=None):
                         continue
                     u = min(max(u, 0.0), ns - 1.001)
                     if aa:
-                        gi = (dxs * p["il_dx"] + dys * p["il_dy"]) / (v * rs) + (dxr * p["il_dx"] + dyr * p["il_dy"]) / (v * rr)
-                        gx = (dxs * p["xl_dx"] + dys * p["xl_dy"]) / (v * rs) + (dxr * p["xl_dx"] + dyr * p["xl_dy"]) / (v * rr)
+                        i_s = 1.0 / (v * rs) if rs > 0 else 0.0
+                        i_r = 1.0 / (v * rr) if rr > 0 else 0.0
+                        gi = (dxs * p["il_dx"] + dys * p["il_dy"]) * i_s + (dxr * p["il_dx"] + dyr * p["il_dy"]) * i_r
+                        gx = (dxs * p["xl_dx"] + dys * p["xl_dy"]) * i_s + (dxr * p["xl_dx"] + dyr * p["xl_dy"]) * i_r
                         L = min(max(p["aa_scale"] * (abs(gi) + abs(gx)) / p["dt"], 1.0), p["aa_max"])
                         a_ = aint[j]
                         xx = u + pad

@@ -17,7 +17,9 @@
  *   processors/migration/migration_engine.py:292-306  grid x = x0 + il*d_il,
  *     y = y0 + xl*d_xl, z = v*tau/2
  *   processors/migration/kirchhoff_migrator.py:378    SPREADING weight
- *     1/(sqrt(r_s^2 r_r^2) + 1e-6)
+ *     1/(sqrt(r_s^2 r_r^2) + 1e-6); OBLIQUITY weight (z/(r_s+e))(z/(r_r+e))
+ *     (the cosine factors of :691-693 alone; mode named in
+ *     models/migration_config.py:40-45)
  *   processors/migration/traveltime_vti.py:164-222    anelliptic eta time per
  *     leg (v0 = v_rms, delta = 0, epsilon = eta)
  *   processors/migration/weights.py:192-236           cosine taper shape
@@ -144,6 +146,8 @@ static int eval_pair(const ctx_t* c, double dxs, double dys, double dxr, double 
     w = (z / (rs + EPS_R)) * (z / (rr + EPS_R)) / (rs * rr + EPS_R);
   else if (p->weight_mode == PSTM_WEIGHT_SPREADING)
     w = 1.0 / (rs * rr + EPS_R);
+  else if (p->weight_mode == PSTM_WEIGHT_OBLIQUITY)
+    w = (z / (rs + EPS_R)) * (z / (rr + EPS_R));
   else
     w = 1.0;
   double tap = 1.0;
@@ -163,8 +167,11 @@ static int eval_pair(const ctx_t* c, double dxs, double dys, double dxr, double 
   o->L = 1.0;
   if (p->flags & PSTM_FLAG_AA) {
     double ex = p->il_dx, ey = p->il_dy, fx = p->xl_dx, fy = p->xl_dy;
-    double g_il = (dxs * ex + dys * ey) / (v * rs) + (dxr * ex + dyr * ey) / (v * rr);
-    double g_xl = (dxs * fx + dys * fy) / (v * rs) + (dxr * fx + dyr * fy) / (v * rr);
+    /* d t / d(image position) per leg; a leg of zero length (tau = 0 and the
+     * source or receiver on the column) has no dip: its term is 0. */
+    double is = rs > 0.0 ? 1.0 / (v * rs) : 0.0, ir = rr > 0.0 ? 1.0 / (v * rr) : 0.0;
+    double g_il = (dxs * ex + dys * ey) * is + (dxr * ex + dyr * ey) * ir;
+    double g_xl = (dxs * fx + dys * fy) * is + (dxr * fx + dyr * fy) * ir;
     double L = p->aa_scale * (fabs(g_il) + fabs(g_xl)) / p->dt;
     if (L < 1.0) L = 1.0;
     if (L > p->aa_max) L = p->aa_max;
@@ -183,6 +190,9 @@ static int eval_pair(const ctx_t* c, double dxs, double dys, double dxr, double 
     } else if (p->weight_mode == PSTM_WEIGHT_SPREADING) {
       double den = rs * rr + EPS_R;
       dw = -(drs * rr + rs * drr) / (den * den);
+    } else if (p->weight_mode == PSTM_WEIGHT_OBLIQUITY) {
+      double D = (rs + EPS_R) * (rr + EPS_R);
+      dw = 2.0 * z * dz / D - (z * z / D) * (drs / (rs + EPS_R) + drr / (rr + EPS_R));
     } else {
       dw = 0.0;
     }

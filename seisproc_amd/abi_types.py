@@ -30,8 +30,9 @@ FLAG_ZERO_OUTSIDE = 1 << 4
 FLAG_TIME_APERTURE = 1 << 5
 
 # weight modes
-WEIGHT_FULL, WEIGHT_SPREADING, WEIGHT_NONE = 0, 1, 2
-WEIGHT_CODES = {"full": WEIGHT_FULL, "spreading": WEIGHT_SPREADING, "none": WEIGHT_NONE}
+WEIGHT_FULL, WEIGHT_SPREADING, WEIGHT_NONE, WEIGHT_OBLIQUITY = 0, 1, 2, 3
+WEIGHT_CODES = {"full": WEIGHT_FULL, "spreading": WEIGHT_SPREADING, "none": WEIGHT_NONE,
+                "obliquity": WEIGHT_OBLIQUITY}
 
 # velocity layouts
 VEL_1D, VEL_3D = 0, 1

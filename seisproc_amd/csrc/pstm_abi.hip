@@ -49,7 +49,7 @@ static int validate(const pstm_params_t* p) {
   if (!(p->max_angle_deg > 0 && p->max_angle_deg <= 90))
     return set_error(PSTM_ERR_INVALID, "max_angle_deg must be in (0, 90]");
   if (p->dtype < PSTM_F32 || p->dtype > PSTM_F16) return set_error(PSTM_ERR_INVALID, "bad dtype %d", p->dtype);
-  if (p->weight_mode < PSTM_WEIGHT_FULL || p->weight_mode > PSTM_WEIGHT_NONE)
+  if (p->weight_mode < PSTM_WEIGHT_FULL || p->weight_mode > PSTM_WEIGHT_OBLIQUITY)
     return set_error(PSTM_ERR_INVALID, "bad weight_mode %d", p->weight_mode);
   if (p->vel_mode != PSTM_VEL_1D && p->vel_mode != PSTM_VEL_3D)
     return set_error(PSTM_ERR_INVALID, "bad vel_mode %d", p->vel_mode);
@@ -68,9 +68,12 @@ static int validate(const pstm_params_t* p) {
     if (!(p->aa_scale >= 0)) return set_error(PSTM_ERR_INVALID, "aa_scale must be >= 0");
     pad = (int)ceil(p->aa_max) + 2;
   }
-  if ((int64_t)p->n_samples + 2 * (pad + kWindowMargin) + 2 > kSampleCap)
-    return set_error(PSTM_ERR_UNSUPPORTED, "n_samples %d too long for the LDS sample buffer (%d)",
-                     p->n_samples, kSampleCap);
+  // A staged window never exceeds n_samples + 2 (pad + margin) + 1 samples;
+  // with the antialias filter each sample is held in FP64 (two floats).
+  const int64_t elem = (p->flags & PSTM_FLAG_AA) ? 2 : 1;
+  if (elem * ((int64_t)p->n_samples + 2 * (pad + kWindowMargin) + 2) > kSampleCap)
+    return set_error(PSTM_ERR_UNSUPPORTED, "n_samples %d too long for the LDS sample buffer (%d floats%s)",
+                     p->n_samples, kSampleCap, elem == 2 ? ", FP64 windows with antialias" : "");
   if ((int64_t)p->n_il * p->n_xl * (int64_t)p->n_tau > ((int64_t)1 << 40))
     return set_error(PSTM_ERR_INVALID, "image too large");
   return PSTM_OK;

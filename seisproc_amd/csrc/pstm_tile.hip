@@ -25,6 +25,8 @@
 // sums (:724-730).  The traveltime is evaluated in the equivalent
 // time form t_leg = sqrt(tau^2/4 + h^2 / v^2), which is r/v with
 // z = v tau / 2 (migration_engine.py:305-306).
+#include <type_traits>
+
 #include "pstm_internal.h"
 
 namespace pstm {
@@ -45,12 +47,37 @@ __device__ __forceinline__ float uniform_f(float v) {
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Inclusive prefix sum over the 64 lanes of a wave.
-__device__ __forceinline__ float wave_scan_f(float v, int lane) {
+template <typename T>
+__device__ __forceinline__ T wave_scan(T v, int lane) {
   for (int off = 1; off < 64; off <<= 1) {
-    float o = __shfl_up(v, off);
+    T o = __shfl_up(v, off);
     if (lane >= off) v += o;
   }
   return v;
+}
+
+// Causal then anticausal running sum of one staged window (one wave).  With
+// the antialias filter the window holds FP64 values: the double integral of
+// an FP32 trace is exact in FP64, and its second difference is what the
+// triangle taps read.
+template <typename T>
+__device__ __forceinline__ void integrate_window(T* w, int len, int lane) {
+  T carry = 0;
+  for (int s0 = 0; s0 < len; s0 += 64) {
+    const int s = s0 + lane;
+    T v = s < len ? w[s] : T(0);
+    v = wave_scan(v, lane) + carry;
+    if (s < len) w[s] = v;
+    carry = __shfl(v, 63);
+  }
+  carry = 0;
+  for (int s0 = 0; s0 < len; s0 += 64) {
+    const int s = len - 1 - (s0 + lane);
+    T v = s >= 0 ? w[s] : T(0);
+    v = wave_scan(v, lane) + carry;
+    if (s >= 0) w[s] = v;
+    carry = __shfl(v, 63);
+  }
 }
 __device__ __forceinline__ int wave_scan_i(int v, int lane) {
   for (int off = 1; off < 64; off <<= 1) {
@@ -89,8 +116,8 @@ __device__ __forceinline__ float taper_of(float hm2, float hl2, float tw) {
   return 0.5f * (1.0f + __cosf(3.14159265358979f * fminf(x, 1.0f)));
 }
 
-struct TileShared {
-  float samp[kSampleCap];
+struct alignas(16) TileShared {
+  alignas(16) float samp[kSampleCap];  // staged windows: FP32, or FP64 with antialias
   int32_t q_j[kQueueCap];
   int32_t q_w0[kQueueCap];
   int32_t q_len[kQueueCap];
@@ -113,8 +140,11 @@ struct PairShared {
 template <int MODE, int TX, int TY, int DT, bool AA, bool VTI>
 __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
   constexpr int NC = TX * TY;
+  using WT = typename std::conditional<AA, double, float>::type;  // staged window element
+  constexpr int WF = (int)(sizeof(WT) / sizeof(float));           // floats per element
   __shared__ TileShared S;
   __shared__ PairShared<NC, AA> P;
+  WT* const Wn = reinterpret_cast<WT*>(S.samp);
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
@@ -249,13 +279,14 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
         if (wave == 0) {
           const int i = qs + lane;
           const bool in = lane < kMaxBatch && i < qn;
-          const int len = in ? S.q_len[i] : 0;
-          const int incl = wave_scan_i(len, lane);
+          const int cost = in ? WF * S.q_len[i] : 0;
+          const int incl = wave_scan_i(cost, lane);
           const bool fit = in && incl <= kSampleCap;
           const uint64_t m = __ballot(fit);
           if (fit) {
-            S.b_off[lane] = incl - len;
-            S.b_base[lane] = incl - len - S.q_w0[i];
+            const int off = (incl - cost) / WF;  // in window elements
+            S.b_off[lane] = off;
+            S.b_base[lane] = off - S.q_w0[i];
           }
           if (lane == 0) S.nb = __popcll(m);
         }
@@ -267,7 +298,7 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
           const int e = qs + b;
           const int w0 = S.q_w0[e], len = S.q_len[e], off = S.b_off[b];
           if constexpr (MODE == MODE_ADJ) {
-            for (int s = lane; s < len; s += 64) S.samp[off + s] = 0.f;
+            for (int s = lane; s < len; s += 64) Wn[off + s] = WT(0);
           } else {
             const int id = a.ids[S.q_j[e]];
             const float sc = a.scale ? a.scale[id] : 1.f;
@@ -276,7 +307,7 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
               const int src = w0 + s;
               float v = 0.f;
               if (src >= 0 && src < a.n_samples) v = load_sample<DT>(a.traces, row + src) * sc;
-              S.samp[off + s] = v;
+              Wn[off + s] = (WT)v;
             }
           }
         }
@@ -292,25 +323,7 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
         __syncthreads();
         if constexpr (AA && MODE != MODE_ADJ) {
           // double integration (causal, then anticausal) of each window
-          for (int b = wave; b < nb; b += kWaves) {
-            const int len = S.q_len[qs + b], off = S.b_off[b];
-            float carry = 0.f;
-            for (int s0 = 0; s0 < len; s0 += 64) {
-              const int s = s0 + lane;
-              float v = s < len ? S.samp[off + s] : 0.f;
-              v = wave_scan_f(v, lane) + carry;
-              if (s < len) S.samp[off + s] = v;
-              carry = __shfl(v, 63);
-            }
-            carry = 0.f;
-            for (int s0 = 0; s0 < len; s0 += 64) {
-              const int s = len - 1 - (s0 + lane);
-              float v = s >= 0 ? S.samp[off + s] : 0.f;
-              v = wave_scan_f(v, lane) + carry;
-              if (s >= 0) S.samp[off + s] = v;
-              carry = __shfl(v, 63);
-            }
-          }
+          for (int b = wave; b < nb; b += kWaves) integrate_window(Wn + S.b_off[b], S.q_len[qs + b], lane);
           __syncthreads();
         }
 
@@ -343,6 +356,7 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
             float w;
             if (a.weight_mode == PSTM_WEIGHT_FULL) w = (tq * q[c]) * prs * prs;
             else if (a.weight_mode == PSTM_WEIGHT_SPREADING) w = q[c] * prs;
+            else if (a.weight_mode == PSTM_WEIGHT_OBLIQUITY) w = tq * prs;
             else w = 1.f;
             if (a.flags & PSTM_FLAG_TAPER) w *= taper_of(hm, hl2[c], a.taper_width);
             float L = 1.f, iL2 = 1.f;
@@ -356,16 +370,21 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
             if constexpr (MODE == MODE_FWD || MODE == MODE_GRAD) {
               const int i0 = (int)u;
               const float f = u - (float)i0;
-              const float a0 = S.samp[base + i0], a1 = S.samp[base + i0 + 1];
-              float amp = fmaf(f, a1 - a0, a0);
+              float amp, a0 = 0.f, a1 = 0.f;
               if constexpr (AA) {
+                // triangle taps on the FP64 double integral
                 const float um = u - L, up = u + L;
                 const int im = (int)floorf(um), ip = (int)up;
-                const float fm = um - (float)im, fp = up - (float)ip;
-                const float b0 = S.samp[base + im], b1 = S.samp[base + im + 1];
-                const float c0 = S.samp[base + ip], c1 = S.samp[base + ip + 1];
-                const float am = fmaf(fm, b1 - b0, b0), ap2 = fmaf(fp, c1 - c0, c0);
-                amp = (2.f * amp - am - ap2) * iL2;
+                const double fd = (double)f, fmd = (double)(um - (float)im), fpd = (double)(up - (float)ip);
+                const double A0 = Wn[base + i0], A1 = Wn[base + i0 + 1];
+                const double B0 = Wn[base + im], B1 = Wn[base + im + 1];
+                const double C0 = Wn[base + ip], C1 = Wn[base + ip + 1];
+                const double ac = fma(fd, A1 - A0, A0), am = fma(fmd, B1 - B0, B0), ap2 = fma(fpd, C1 - C0, C0);
+                amp = (float)(2.0 * ac - am - ap2) * iL2;
+              } else {
+                a0 = Wn[base + i0];
+                a1 = Wn[base + i0 + 1];
+                amp = fmaf(f, a1 - a0, a0);
               }
               if constexpr (MODE == MODE_FWD) {
                 accb[c] += ok ? w * amp : 0.f;
@@ -378,6 +397,7 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
                 if (a.weight_mode == PSTM_WEIGHT_FULL) dw = -2.f * sqv * w * (1.f - q[c] * (hsr + hrr));
                 else if (a.weight_mode == PSTM_WEIGHT_SPREADING)
                   dw = -2.f * sqv * w * (1.f - 0.5f * q[c] * (hsr + hrr));
+                else if (a.weight_mode == PSTM_WEIGHT_OBLIQUITY) dw = sqv * w * q[c] * (hsr + hrr);
                 else dw = 0.f;
                 const bool clamped = !(u_raw >= 0.f && u_raw <= a.umax);
                 const float dt_dv = -sqv * q[c] * fmaf(h2.x, rs, h2.y * rr);
@@ -393,18 +413,18 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
                   const float um = u - L, up = u + L;
                   const int i0 = (int)u, im = (int)floorf(um), ip = (int)up;
                   const float f = u - (float)i0, fm = um - (float)im, fp = up - (float)ip;
-                  const float gc = 2.f * gsc * iL2, ge = -gsc * iL2;
-                  atomicAdd(&S.samp[base + i0], gc * (1.f - f));
-                  atomicAdd(&S.samp[base + i0 + 1], gc * f);
-                  atomicAdd(&S.samp[base + im], ge * (1.f - fm));
-                  atomicAdd(&S.samp[base + im + 1], ge * fm);
-                  atomicAdd(&S.samp[base + ip], ge * (1.f - fp));
-                  atomicAdd(&S.samp[base + ip + 1], ge * fp);
+                  const double gc = 2.0 * (double)(gsc * iL2), ge = -(double)(gsc * iL2);
+                  atomicAdd(&Wn[base + i0], (WT)(gc * (1.0 - f)));
+                  atomicAdd(&Wn[base + i0 + 1], (WT)(gc * f));
+                  atomicAdd(&Wn[base + im], (WT)(ge * (1.0 - fm)));
+                  atomicAdd(&Wn[base + im + 1], (WT)(ge * fm));
+                  atomicAdd(&Wn[base + ip], (WT)(ge * (1.0 - fp)));
+                  atomicAdd(&Wn[base + ip + 1], (WT)(ge * fp));
                 } else {
                   const int i0 = (int)u;
                   const float f = u - (float)i0;
-                  atomicAdd(&S.samp[base + i0], gsc * (1.f - f));
-                  atomicAdd(&S.samp[base + i0 + 1], gsc * f);
+                  atomicAdd(&Wn[base + i0], (WT)(gsc * (1.f - f)));
+                  atomicAdd(&Wn[base + i0 + 1], (WT)(gsc * f));
                 }
               }
             }
@@ -429,25 +449,8 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
         if constexpr (MODE == MODE_ADJ) {
           if constexpr (AA) {
             // transpose of the double integration: the same symmetric operator
-            for (int b = wave; b < nb; b += kWaves) {
-              const int len = S.q_len[qs + b], off = S.b_off[b];
-              float carry = 0.f;
-              for (int s0 = 0; s0 < len; s0 += 64) {
-                const int s = s0 + lane;
-                float v = s < len ? S.samp[off + s] : 0.f;
-                v = wave_scan_f(v, lane) + carry;
-                if (s < len) S.samp[off + s] = v;
-                carry = __shfl(v, 63);
-              }
-              carry = 0.f;
-              for (int s0 = 0; s0 < len; s0 += 64) {
-                const int s = len - 1 - (s0 + lane);
-                float v = s >= 0 ? S.samp[off + s] : 0.f;
-                v = wave_scan_f(v, lane) + carry;
-                if (s >= 0) S.samp[off + s] = v;
-                carry = __shfl(v, 63);
-              }
-            }
+            for (int b = wave; b < nb; b += kWaves) integrate_window(Wn + S.b_off[b], S.q_len[qs + b], lane);
+            __syncthreads();
           }
           // flush the windows into the output traces
           for (int b = wave; b < nb; b += kWaves) {
@@ -457,7 +460,7 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
             float* row = a.traces_out + (int64_t)id * a.n_samples;
             for (int s = lane; s < len; s += 64) {
               const int dst = w0 + s;
-              const float v = S.samp[off + s];
+              const float v = (float)Wn[off + s];
               if (dst >= 0 && dst < a.n_samples && v != 0.f) atomicAdd(&row[dst], v);
             }
           }

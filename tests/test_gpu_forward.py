@@ -54,6 +54,7 @@ CASES = [
                                                  weight_mode=WEIGHT_NONE)),
     ("utm-coordinates", dict(utm=True), dict()),
     ("kahan", dict(), dict(flags=FLAG_KAHAN)),
+    ("obliquity", dict(lateral=True), dict(weight_mode=3)),
 ]
 
 

@@ -202,7 +202,7 @@ def test_vti_forward_equals_isotropic_at_eta_zero(oracle_lib):
 # ---------------------------------------------------------------- brute force
 @pytest.mark.parametrize("flags,lateral,wm", [
     (0, False, 0), (FLAG_AA, False, 0), (FLAG_AA | FLAG_VTI, True, 0), (FLAG_TAPER, True, 1),
-    (FLAG_TIME_APERTURE | FLAG_ZERO_OUTSIDE, False, 2),
+    (FLAG_TIME_APERTURE | FLAG_ZERO_OUTSIDE, False, 2), (0, True, 3),
 ])
 def test_oracle_matches_numpy_brute_force(oracle_lib, flags, lateral, wm):
     s = S.tiny_survey(seed=4, n_bins=5, n_classes=2, n_samples=121, n_img=3, aperture=250.0, flags=flags,
@@ -277,7 +277,7 @@ def test_diffractor_focuses_at_tau0(oracle_lib):
 
 
 # ---------------------------------------------------------------- velocity gradient
-@pytest.mark.parametrize("wm", [0, 1])
+@pytest.mark.parametrize("wm", [0, 1, 3])
 def test_oracle_vel_grad_central_differences(oracle_lib, wm):
     s = S.tiny_survey(seed=6, n_bins=6, n_classes=2, n_samples=121, n_img=3, aperture=300.0, lateral=True)
     sx, sy, rx, ry = s.numpy_geometry()
@@ -298,3 +298,10 @@ def test_oracle_vel_grad_central_differences(oracle_lib, wm):
     fd = (Jp - Jm) / (2 * h)
     an = float(np.sum(grad * delta))
     assert abs(fd - an) <= 1e-5 * abs(an)
+
+
+def test_antialias_zero_length_leg_is_finite(oracle_lib):
+    # tau = 0 with the source/receiver on the column: no dip, L = 1, finite image
+    p = _p(n_samples=50, n_tau=3, flags=FLAG_AA, aa_max=8.0)
+    img, fold = oracle.forward(p, np.arange(50.0)[None, :], [0.0], [0.0], [0.0], [0.0], np.full(3, 2000.0))
+    assert np.all(np.isfinite(img)) and fold[0, 0, 0] == 1
