@@ -15,7 +15,7 @@ columns.
 
 The CPU baseline is the torch-CPU restatement of the reference's CPU method
 (oracle/ref_torch.py), timed on a bounded sample of the same workload: one
-100-point reference tile (10 x 10 image columns) at every 8th output time,
+100-point reference tile (10 x 10 image columns) at every 16th output time,
 with every trace that reaches it.
 """
 from __future__ import annotations
@@ -37,31 +37,17 @@ HBM_PEAK_GBS = 8000.0
 def _parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cfg3", choices=["cfg3", "cfg2", "cfg5"])
     ap.add_argument("--classes", type=int, default=None, help="offset classes (cfg3: 25)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
-    ap.add_argument("--cpu-tau-stride", type=int, default=8)
+    ap.add_argument("--cpu-tau-stride", type=int, default=16)
+    ap.add_argument("--forms", default=None, help="comma list of trace forms to time (default: all)")
+    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
+                    help="first run a plain device copy of this many GiB (counter calibration)")
     return ap.parse_args()
-
-
-def _timed(fn, steps, torch):
-    """Time `steps` calls of fn on the current stream: HIP events bracket the
-    launches (kernel-time per launch) and synchronize() brackets the wall."""
-    s = torch.cuda.current_stream()
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record(s)
-    for _ in range(steps):
-        fn()
-    e1.record(s)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    return wall / steps, e0.elapsed_time(e1) / 1e3 / steps
 
 
 def _workload(args, torch):
@@ -175,7 +161,16 @@ def main():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(1)
     t_start = time.perf_counter()
+    if args.copy_probe_gib > 0:
+        n = int(args.copy_probe_gib * (1 << 30)) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     s, cfg, forms = _workload(args, torch)
+    if args.forms:
+        forms = [f for f in args.forms.split(",") if f]
     plan = PstmPlan(s.params, s.src_xy, s.rcv_xy)
     vel = s.vel.contiguous()
     eta = s.eta
@@ -201,24 +196,31 @@ def main():
 
     main_form = forms[0]
     results, images = {}, {}
+    stream = torch.cuda.current_stream()
     for form in forms:
-        w = args.warmup if form == main_form else 1
+        w = args.warmup if form == main_form else 0
         k = args.steps if form == main_form else 1
+        tr, sc = data[form]
         for _ in range(w):
             step(form)
         torch.cuda.synchronize()
-        contrib = float(fold.double().sum())
-        wall, kern = _timed(lambda: step(form), k, torch)
-        # the zero_() fills are outside the kernel timing: time the kernel alone
-        tr, sc = data[form]
-        _, kern_only = _timed(lambda: plan.forward(tr, vel, scale=sc, eta=eta, image=image, comp=comp,
-                                                   fold=fold), 1, torch)
-        step(form)
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(k)]
+        t0 = time.perf_counter()
+        for e0, e1 in ev:
+            image.zero_()
+            comp.zero_()
+            fold.zero_()
+            e0.record(stream)
+            plan.forward(tr, vel, scale=sc, eta=eta, image=image, comp=comp, fold=fold)
+            e1.record(stream)
         torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / k
+        kern = sum(e0.elapsed_time(e1) for e0, e1 in ev) / 1e3 / k
+        contrib = float(fold.double().sum())
         images[form] = (image + comp).cpu()
-        results[form] = {"wall_s": wall, "kernel_s": kern_only, "contributions": contrib,
+        results[form] = {"wall_s": wall, "kernel_s": kern, "contributions": contrib,
                          "gcontrib_s": contrib / wall / 1e9}
-        print(f"[bench] {form}: {wall * 1e3:.1f} ms/step, kernel {kern_only * 1e3:.1f} ms, "
+        print(f"[bench] {form}: {wall * 1e3:.1f} ms/step, kernel {kern * 1e3:.1f} ms, "
               f"{contrib / wall / 1e9:.1f} Gcontrib/s", file=sys.stderr, flush=True)
 
     err, err_desc = ({}, None)
