@@ -17,7 +17,8 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT_DIR = HERE / "_obj"
 LIB = HERE / "libpstm_hip.so"
-SOURCES = ["pstm_abi.hip", "pstm_plan.hip", "pstm_tile.hip"]
+SOURCES = ["pstm_abi.hip", "pstm_plan.hip", "pstm_fwd_f32.hip", "pstm_fwd_bf16.hip", "pstm_fwd_f16.hip",
+           "pstm_adj.hip", "pstm_grad.hip"]
 ARCH = os.environ.get("PSTM_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -1,0 +1,34 @@
+// pstm_adj.hip -- adjoint (demigration) tile kernels and the dispatch of all
+// three tile operators.
+#include "pstm_tile.h"
+
+namespace pstm {
+
+int launch_forward(const pstm_params_t* p, const PlanLayout& L, const TileArgs& a, hipStream_t s) {
+  int rc;
+  if (p->dtype == PSTM_F32) rc = launch_forward_dt<PSTM_F32>(p, L, a, s);
+  else if (p->dtype == PSTM_BF16) rc = launch_forward_dt<PSTM_BF16>(p, L, a, s);
+  else rc = launch_forward_dt<PSTM_F16>(p, L, a, s);
+  return rc != PSTM_OK ? rc : check_launch("forward");
+}
+
+int launch_adjoint(const pstm_params_t* p, const PlanLayout& L, const TileArgs& a, hipStream_t s) {
+  const bool aa = (p->flags & PSTM_FLAG_AA) != 0, vti = (p->flags & PSTM_FLAG_VTI) != 0;
+  const bool gen = !fast_path(p);
+  int rc;
+#define PSTM_ADJ(TX, TY, AAV, VTIV, GENV) launch_one<MODE_ADJ, TX, TY, PSTM_F32, AAV, VTIV, GENV>(a, s)
+#define PSTM_ADJ4(TX, TY, GENV)                         \
+  (aa ? (vti ? PSTM_ADJ(TX, TY, true, true, GENV)       \
+             : PSTM_ADJ(TX, TY, true, false, GENV))     \
+      : (vti ? PSTM_ADJ(TX, TY, false, true, GENV)      \
+             : PSTM_ADJ(TX, TY, false, false, GENV)))
+  if (L.tile_x == 4 && L.tile_y == 4) rc = gen ? PSTM_ADJ4(4, 4, true) : PSTM_ADJ4(4, 4, false);
+  else if (L.tile_x == 16 && L.tile_y == 1) rc = PSTM_ADJ4(16, 1, true);
+  else if (L.tile_x == 1 && L.tile_y == 16) rc = PSTM_ADJ4(1, 16, true);
+  else rc = set_error(PSTM_ERR_INVALID, "unsupported tile shape %dx%d", L.tile_x, L.tile_y);
+#undef PSTM_ADJ4
+#undef PSTM_ADJ
+  return rc != PSTM_OK ? rc : check_launch("adjoint");
+}
+
+}  // namespace pstm

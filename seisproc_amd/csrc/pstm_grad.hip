@@ -1,0 +1,23 @@
+// pstm_grad.hip -- velocity-gradient tile kernels (isotropic, no antialias,
+// general path).
+#include "pstm_tile.h"
+
+namespace pstm {
+
+int launch_vel_grad(const pstm_params_t* p, const PlanLayout& L, const TileArgs& a, hipStream_t s) {
+  int rc;
+#define PSTM_GRAD(TX, TY, DTV) launch_one<MODE_GRAD, TX, TY, DTV, false, false, true>(a, s)
+#define PSTM_GRAD3(TX, TY)                                                      \
+  (p->dtype == PSTM_F32 ? PSTM_GRAD(TX, TY, PSTM_F32)                           \
+                        : p->dtype == PSTM_BF16 ? PSTM_GRAD(TX, TY, PSTM_BF16)   \
+                                                : PSTM_GRAD(TX, TY, PSTM_F16))
+  if (L.tile_x == 4 && L.tile_y == 4) rc = PSTM_GRAD3(4, 4);
+  else if (L.tile_x == 16 && L.tile_y == 1) rc = PSTM_GRAD3(16, 1);
+  else if (L.tile_x == 1 && L.tile_y == 16) rc = PSTM_GRAD3(1, 16);
+  else rc = set_error(PSTM_ERR_INVALID, "unsupported tile shape %dx%d", L.tile_x, L.tile_y);
+#undef PSTM_GRAD3
+#undef PSTM_GRAD
+  return rc != PSTM_OK ? rc : check_launch("vel_grad");
+}
+
+}  // namespace pstm

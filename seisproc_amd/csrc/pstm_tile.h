@@ -1,5 +1,5 @@
-// pstm_tile.hip -- gfx950 Kirchhoff PSTM tile kernels (forward, adjoint,
-// velocity gradient).
+// pstm_tile.h -- gfx950 Kirchhoff PSTM tile kernels (forward, adjoint,
+// velocity gradient), included by the per-mode translation units.
 //
 // Work decomposition (one workgroup = 4 wave64 = 256 threads):
 //   * a tile of TX x TY image columns and 256 consecutive output times;
@@ -13,11 +13,13 @@
 //   * each queued trace gets a sample window bounded rigorously from the DSR
 //     traveltime over the tile's columns and times; sub-batches of windows
 //     are staged HBM -> LDS with coalesced loads (FP32/BF16/FP16 converted to
-//     FP32 with the per-trace scale applied), integrated twice in LDS when
-//     the triangle antialias filter is on;
+//     FP32 with the per-trace scale applied), integrated twice in FP64 in LDS
+//     when the triangle antialias filter is on;
 //   * per (trace, column) the source/receiver horizontal distances are
 //     computed once into LDS; each lane then evaluates the DSR time, masks,
-//     weight and interpolation for its own output time.
+//     weight and interpolation for its own output time.  The common case
+//     (FULL weight, no taper, clamped record end) is compiled branch-free
+//     across the tile's columns so their dependency chains interleave.
 //
 // Reference method: processors/migration/kirchhoff_kernel.py:459-854
 // (migrate_kirchhoff_full) -- same masks (:635, :670-685), DSR time (:688),
@@ -25,6 +27,8 @@
 // sums (:724-730).  The traveltime is evaluated in the equivalent
 // time form t_leg = sqrt(tau^2/4 + h^2 / v^2), which is r/v with
 // z = v tau / 2 (migration_engine.py:305-306).
+#pragma once
+
 #include <type_traits>
 
 #include "pstm_internal.h"
@@ -79,13 +83,6 @@ __device__ __forceinline__ void integrate_window(T* w, int len, int lane) {
     carry = __shfl(v, 63);
   }
 }
-__device__ __forceinline__ int wave_scan_i(int v, int lane) {
-  for (int off = 1; off < 64; off <<= 1) {
-    int o = __shfl_up(v, off);
-    if (lane >= off) v += o;
-  }
-  return v;
-}
 
 template <int DT>
 __device__ __forceinline__ float load_sample(const void* tr, int64_t idx) {
@@ -121,8 +118,8 @@ struct alignas(16) TileShared {
   int32_t q_j[kQueueCap];
   int32_t q_w0[kQueueCap];
   int32_t q_len[kQueueCap];
-  int32_t b_base[kMaxBatch];   // LDS index of sample 0 of the trace (off - w0)
-  int32_t b_off[kMaxBatch];    // LDS offset of the window
+  int32_t b_base[kMaxBatch];   // window-element index of sample 0 of the trace (off - w0)
+  int32_t b_off[kMaxBatch];    // window-element offset of the window
   int32_t scan[kWaves];
   int32_t queue_n;
   int32_t nb;
@@ -132,12 +129,14 @@ struct alignas(16) TileShared {
 };
 
 template <int NC, bool AA>
-struct PairShared {
+struct alignas(16) PairShared {
   float2 h2[kMaxBatch][NC];
   float4 g[AA ? kMaxBatch : 1][AA ? NC : 1];
 };
 
-template <int MODE, int TX, int TY, int DT, bool AA, bool VTI>
+// GEN = false: FULL weight, no taper, clamped record end (compile-time);
+// GEN = true: weight mode, taper and zero-outside read from the arguments.
+template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN>
 __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
   constexpr int NC = TX * TY;
   using WT = typename std::conditional<AA, double, float>::type;  // staged window element
@@ -170,19 +169,17 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
   [[maybe_unused]] float k1[VTI ? NC : 1], m2[VTI ? NC : 1];
   float emax_l = 0.f;
   float qmin_l = 3.0e38f, qmax_l = 0.f, hmax_l = -1.f;
-  float colX[NC], colY[NC];
-  bool cval[NC];
+  unsigned cvalid = 0;  // bit c: column c of the tile is inside the image
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const int il = tx * TX + c / TY, xl = ty * TY + c % TY;
-    cval[c] = il < a.n_il && xl < a.n_xl;
-    colX[c] = (float)il * a.d_il;
-    colY[c] = (float)xl * a.d_xl;
+    const bool cv = il < a.n_il && xl < a.n_xl;
+    cvalid |= cv ? (1u << c) : 0u;
     const int64_t vi = a.vel_mode == PSTM_VEL_1D
                            ? (int64_t)kk
-                           : ((int64_t)(cval[c] ? il : 0) * a.n_xl + (cval[c] ? xl : 0)) * a.n_tau + kk;
+                           : ((int64_t)(cv ? il : 0) * a.n_xl + (cv ? xl : 0)) * a.n_tau + kk;
     const float v = a.vel[vi];
-    const bool ok = kvalid && cval[c] && v > 0.f;
+    const bool ok = kvalid && cv && v > 0.f;
     const float vv = ok ? v : 1.f;
     q[c] = 1.0f / (vv * vv);
     const float z = 0.5f * vv * tau;
@@ -247,9 +244,13 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const int il = tx * TX + c / TY, xl = ty * TY + c % TY;
-      m_in[c] = (kvalid && cval[c]) ? a.image_in[((int64_t)il * a.n_xl + xl) * a.n_tau + k] : 0.f;
+      m_in[c] = (kvalid && ((cvalid >> c) & 1u)) ? a.image_in[((int64_t)il * a.n_xl + xl) * a.n_tau + k] : 0.f;
     }
   }
+
+  const bool wm_full = GEN ? a.weight_mode == PSTM_WEIGHT_FULL : true;
+  const bool use_taper = GEN ? (a.flags & PSTM_FLAG_TAPER) != 0 : false;
+  const bool zero_out = GEN ? (a.flags & PSTM_FLAG_ZERO_OUTSIDE) != 0 : false;
 
   if (hblk > 0.f) {
     // tile rectangle over valid columns (local coordinates)
@@ -280,7 +281,7 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
           const int i = qs + lane;
           const bool in = lane < kMaxBatch && i < qn;
           const int cost = in ? WF * S.q_len[i] : 0;
-          const int incl = wave_scan_i(cost, lane);
+          const int incl = wave_scan(cost, lane);
           const bool fit = in && incl <= kSampleCap;
           const uint64_t m = __ballot(fit);
           if (fit) {
@@ -330,17 +331,22 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
         // -- contributions
         for (int b = 0; b < nb; ++b) {
           const int base = uniform_i(S.b_base[b]);
+          float2 h2[NC];
+          bool any = false;
 #pragma unroll
           for (int c = 0; c < NC; ++c) {
-            const float2 h2 = P.h2[b][c];
-            const float hm = uniform_f(fmaxf(h2.x, h2.y));
-            if (!(hm < hcol[c])) continue;
-            const bool pass = hm < hl2[c];
-            const float xs = fmaf(h2.x, q[c], tqx), xr = fmaf(h2.y, q[c], tqx);
+            h2[c] = P.h2[b][c];
+            any |= fmaxf(h2[c].x, h2[c].y) < hcol[c];
+          }
+          if (!any) continue;  // the trace reaches none of this wave's samples
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            const float hm = fmaxf(h2[c].x, h2[c].y);
+            const float xs = fmaf(h2[c].x, q[c], tqx), xr = fmaf(h2[c].y, q[c], tqx);
             const float rs = __builtin_amdgcn_rsqf(xs), rr = __builtin_amdgcn_rsqf(xr);
             float t;
             if constexpr (VTI) {
-              const float ps = h2.x * q[c], pr = h2.y * q[c];
+              const float ps = h2[c].x * q[c], pr = h2[c].y * q[c];
               const float ds = fmaf(k1[c], ps, tqx), dr = fmaf(k1[c], pr, tqx);
               const float vs = fmaf(m2[c] * ps, ps * __builtin_amdgcn_rcpf(ds), tqx + ps);
               const float vr = fmaf(m2[c] * pr, pr * __builtin_amdgcn_rcpf(dr), tqx + pr);
@@ -349,16 +355,21 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
               t = fmaf(xs, rs, xr * rr);
             }
             const float u_raw = fmaf(t, a.inv_dt, -a.t0_over_dt);
-            bool ok = pass;
-            if (a.flags & PSTM_FLAG_ZERO_OUTSIDE) ok = ok && (u_raw >= 0.f) && (u_raw <= a.u_last);
+            bool ok = hm < hl2[c];
+            if (zero_out) ok = ok && (u_raw >= 0.f) && (u_raw <= a.u_last);
             const float u = __builtin_amdgcn_fmed3f(u_raw, 0.f, a.umax);
             const float prs = rs * rr;
             float w;
-            if (a.weight_mode == PSTM_WEIGHT_FULL) w = (tq * q[c]) * prs * prs;
-            else if (a.weight_mode == PSTM_WEIGHT_SPREADING) w = q[c] * prs;
-            else if (a.weight_mode == PSTM_WEIGHT_OBLIQUITY) w = tq * prs;
-            else w = 1.f;
-            if (a.flags & PSTM_FLAG_TAPER) w *= taper_of(hm, hl2[c], a.taper_width);
+            if (wm_full) {
+              w = (tq * q[c]) * (prs * prs);
+            } else if (a.weight_mode == PSTM_WEIGHT_SPREADING) {
+              w = q[c] * prs;
+            } else if (a.weight_mode == PSTM_WEIGHT_OBLIQUITY) {
+              w = tq * prs;
+            } else {
+              w = 1.f;
+            }
+            if (use_taper) w *= taper_of(hm, hl2[c], a.taper_width);
             float L = 1.f, iL2 = 1.f;
             if constexpr (AA) {
               const float4 g = P.g[b][c];
@@ -367,9 +378,9 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
               L = __builtin_amdgcn_fmed3f(L, 1.f, a.aa_max);
               iL2 = __builtin_amdgcn_rcpf(L * L);
             }
+            const int i0 = (int)u;  // u >= 0: truncation is floor
+            const float f = __builtin_amdgcn_fractf(u);
             if constexpr (MODE == MODE_FWD || MODE == MODE_GRAD) {
-              const int i0 = (int)u;
-              const float f = u - (float)i0;
               float amp, a0 = 0.f, a1 = 0.f;
               if constexpr (AA) {
                 // triangle taps on the FP64 double integral
@@ -387,12 +398,12 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
                 amp = fmaf(f, a1 - a0, a0);
               }
               if constexpr (MODE == MODE_FWD) {
-                accb[c] += ok ? w * amp : 0.f;
+                accb[c] = fmaf(ok ? w : 0.f, amp, accb[c]);
                 fcnt[c] += ok ? 1 : 0;
               } else {
                 // d/dv of w * amp with masks held (isotropic, no antialias)
                 const float sqv = __builtin_amdgcn_sqrtf(q[c]);  // 1/v
-                const float hsr = h2.x * rs * rs, hrr = h2.y * rr * rr;
+                const float hsr = h2[c].x * rs * rs, hrr = h2[c].y * rr * rr;
                 float dw;
                 if (a.weight_mode == PSTM_WEIGHT_FULL) dw = -2.f * sqv * w * (1.f - q[c] * (hsr + hrr));
                 else if (a.weight_mode == PSTM_WEIGHT_SPREADING)
@@ -400,7 +411,7 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
                 else if (a.weight_mode == PSTM_WEIGHT_OBLIQUITY) dw = sqv * w * q[c] * (hsr + hrr);
                 else dw = 0.f;
                 const bool clamped = !(u_raw >= 0.f && u_raw <= a.umax);
-                const float dt_dv = -sqv * q[c] * fmaf(h2.x, rs, h2.y * rr);
+                const float dt_dv = -sqv * q[c] * fmaf(h2[c].x, rs, h2[c].y * rr);
                 const float slope = clamped ? 0.f : (a1 - a0);
                 const float d = fmaf(w * slope, dt_dv * a.inv_dt, dw * amp);
                 accb[c] += ok ? d : 0.f;
@@ -411,8 +422,8 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
               if (gsc != 0.f) {
                 if constexpr (AA) {
                   const float um = u - L, up = u + L;
-                  const int i0 = (int)u, im = (int)floorf(um), ip = (int)up;
-                  const float f = u - (float)i0, fm = um - (float)im, fp = up - (float)ip;
+                  const int im = (int)floorf(um), ip = (int)up;
+                  const float fm = um - (float)im, fp = up - (float)ip;
                   const double gc = 2.0 * (double)(gsc * iL2), ge = -(double)(gsc * iL2);
                   atomicAdd(&Wn[base + i0], (WT)(gc * (1.0 - f)));
                   atomicAdd(&Wn[base + i0 + 1], (WT)(gc * f));
@@ -421,8 +432,6 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
                   atomicAdd(&Wn[base + ip], (WT)(ge * (1.0 - fp)));
                   atomicAdd(&Wn[base + ip + 1], (WT)(ge * fp));
                 } else {
-                  const int i0 = (int)u;
-                  const float f = u - (float)i0;
                   atomicAdd(&Wn[base + i0], (WT)(gsc * (1.f - f)));
                   atomicAdd(&Wn[base + i0 + 1], (WT)(gsc * f));
                 }
@@ -561,7 +570,7 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
   if (!kvalid) return;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    if (!cval[c]) continue;
+    if (!((cvalid >> c) & 1u)) continue;
     const int il = tx * TX + c / TY, xl = ty * TY + c % TY;
     const int64_t idx = ((int64_t)il * a.n_xl + xl) * a.n_tau + k;
     if constexpr (MODE == MODE_FWD) {
@@ -581,65 +590,54 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Host-side dispatch
+// Host-side launch of one instantiation.
 
-template <int MODE, int TX, int TY>
-static int launch_shape(const pstm_params_t* p, const TileArgs& a, hipStream_t s) {
+inline int64_t tile_blocks(const TileArgs& a) {
   const int n_tiles = a.n_tiles_x * a.n_tiles_y;
   const int chunk = (n_tiles + 7) / 8;
-  const int64_t blocks = (int64_t)8 * chunk * a.n_tblocks;
+  return (int64_t)8 * chunk * a.n_tblocks;
+}
+
+template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN>
+int launch_one(const TileArgs& a, hipStream_t s) {
+  const int64_t blocks = tile_blocks(a);
   if (blocks <= 0) return PSTM_OK;
   if (blocks > 0x7fffffff) return set_error(PSTM_ERR_INVALID, "too many tiles (%lld)", (long long)blocks);
-  const bool aa = (p->flags & PSTM_FLAG_AA) != 0, vti = (p->flags & PSTM_FLAG_VTI) != 0;
-  const int dt = p->dtype;
-  dim3 g((unsigned)blocks), b(kThreads);
-#define PSTM_LAUNCH(DTV, AAV, VTIV) \
-  hipLaunchKernelGGL((k_tile<MODE, TX, TY, DTV, AAV, VTIV>), g, b, 0, s, a)
-#define PSTM_DT(AAV, VTIV)                                   \
-  do {                                                       \
-    if (dt == PSTM_F32) PSTM_LAUNCH(PSTM_F32, AAV, VTIV);    \
-    else if (dt == PSTM_BF16) PSTM_LAUNCH(PSTM_BF16, AAV, VTIV); \
-    else PSTM_LAUNCH(PSTM_F16, AAV, VTIV);                   \
-  } while (0)
-  if constexpr (MODE == MODE_GRAD) {
-    if (dt == PSTM_F32) PSTM_LAUNCH(PSTM_F32, false, false);
-    else if (dt == PSTM_BF16) PSTM_LAUNCH(PSTM_BF16, false, false);
-    else PSTM_LAUNCH(PSTM_F16, false, false);
-  } else if constexpr (MODE == MODE_ADJ) {
-    if (aa && vti) PSTM_LAUNCH(PSTM_F32, true, true);
-    else if (aa) PSTM_LAUNCH(PSTM_F32, true, false);
-    else if (vti) PSTM_LAUNCH(PSTM_F32, false, true);
-    else PSTM_LAUNCH(PSTM_F32, false, false);
-  } else {
-    if (aa && vti) PSTM_DT(true, true);
-    else if (aa) PSTM_DT(true, false);
-    else if (vti) PSTM_DT(false, true);
-    else PSTM_DT(false, false);
-  }
-#undef PSTM_DT
-#undef PSTM_LAUNCH
+  hipLaunchKernelGGL((k_tile<MODE, TX, TY, DT, AA, VTI, GEN>), dim3((unsigned)blocks), dim3(kThreads), 0, s, a);
   return PSTM_OK;
 }
 
-template <int MODE>
-static int launch_mode(const pstm_params_t* p, const PlanLayout& L, const TileArgs& a, hipStream_t s) {
-  int rc;
-  if (L.tile_x == 4 && L.tile_y == 4) rc = launch_shape<MODE, 4, 4>(p, a, s);
-  else if (L.tile_x == 16 && L.tile_y == 1) rc = launch_shape<MODE, 16, 1>(p, a, s);
-  else if (L.tile_x == 1 && L.tile_y == 16) rc = launch_shape<MODE, 1, 16>(p, a, s);
-  else return set_error(PSTM_ERR_INVALID, "unsupported tile shape %dx%d", L.tile_x, L.tile_y);
-  if (rc != PSTM_OK) return rc;
-  return check_launch(MODE == MODE_FWD ? "forward" : MODE == MODE_ADJ ? "adjoint" : "vel_grad");
+// Whether a call can use the compile-time fast path (GEN = false).
+inline bool fast_path(const pstm_params_t* p) {
+  return p->weight_mode == PSTM_WEIGHT_FULL && !(p->flags & (PSTM_FLAG_TAPER | PSTM_FLAG_ZERO_OUTSIDE));
 }
 
-int launch_forward(const pstm_params_t* p, const PlanLayout& L, const TileArgs& a, hipStream_t s) {
-  return launch_mode<MODE_FWD>(p, L, a, s);
-}
-int launch_adjoint(const pstm_params_t* p, const PlanLayout& L, const TileArgs& a, hipStream_t s) {
-  return launch_mode<MODE_ADJ>(p, L, a, s);
-}
-int launch_vel_grad(const pstm_params_t* p, const PlanLayout& L, const TileArgs& a, hipStream_t s) {
-  return launch_mode<MODE_GRAD>(p, L, a, s);
+// Per-dtype forward dispatch (defined in pstm_fwd_<dtype>.hip).
+template <int DT>
+int launch_forward_dt(const pstm_params_t* p, const PlanLayout& L, const TileArgs& a, hipStream_t s);
+
+}  // namespace pstm
+
+namespace pstm {
+
+// Forward dispatch over tile shape, antialias, VTI and fast/general path for
+// one trace dtype.  The line shapes (2D images) use the general path only.
+template <int DT>
+int launch_forward_impl(const pstm_params_t* p, const PlanLayout& L, const TileArgs& a, hipStream_t s) {
+  const bool aa = (p->flags & PSTM_FLAG_AA) != 0, vti = (p->flags & PSTM_FLAG_VTI) != 0;
+  const bool gen = !fast_path(p);
+#define PSTM_FWD(TX, TY, AAV, VTIV, GENV) launch_one<MODE_FWD, TX, TY, DT, AAV, VTIV, GENV>(a, s)
+#define PSTM_FWD4(TX, TY, GENV)                         \
+  (aa ? (vti ? PSTM_FWD(TX, TY, true, true, GENV)       \
+             : PSTM_FWD(TX, TY, true, false, GENV))     \
+      : (vti ? PSTM_FWD(TX, TY, false, true, GENV)      \
+             : PSTM_FWD(TX, TY, false, false, GENV)))
+  if (L.tile_x == 4 && L.tile_y == 4) return gen ? PSTM_FWD4(4, 4, true) : PSTM_FWD4(4, 4, false);
+  if (L.tile_x == 16 && L.tile_y == 1) return PSTM_FWD4(16, 1, true);
+  if (L.tile_x == 1 && L.tile_y == 16) return PSTM_FWD4(1, 16, true);
+#undef PSTM_FWD4
+#undef PSTM_FWD
+  return set_error(PSTM_ERR_INVALID, "unsupported tile shape %dx%d", L.tile_x, L.tile_y);
 }
 
 }  // namespace pstm
