@@ -29,12 +29,11 @@ constexpr int kWindowMargin = 2;
 // deterministically from the job parameters on the host.
 struct PlanLayout {
   int64_t n_traces;
-  int32_t ncx, ncy;      // spatial cells (along il, along xl)
+  int32_t ncx, ncy;      // spatial cells of 4 x 4 image columns (along il, along xl)
   int32_t key_bits;      // radix-sort key width
   float cell_x0, cell_y0;  // local coordinate of the cell grid corner (m)
   float cell_w, cell_h;    // cell size (m)
   float d_il, d_xl;        // column spacing (m)
-  int32_t tile_x, tile_y;  // columns per tile along il / xl
   double ux_x, ux_y;       // unit vector of the inline axis (world)
   double uy_x, uy_y;       // unit vector of the crossline axis (world)
   // byte offsets inside the workspace
@@ -56,6 +55,7 @@ struct PlanHeader {
 struct TileArgs {
   // sizes
   int32_t n_samples, n_il, n_xl, n_tau;
+  int32_t tile_x, tile_y;  // image columns per tile along il / xl
   int32_t n_tiles_x, n_tiles_y, n_tblocks;
   // time axes
   float inv_dt, t0_over_dt, umax, u_last;  // umax = n-1.001, u_last = n-1

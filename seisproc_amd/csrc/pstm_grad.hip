@@ -11,10 +11,11 @@ int launch_vel_grad(const pstm_params_t* p, const PlanLayout& L, const TileArgs&
   (p->dtype == PSTM_F32 ? PSTM_GRAD(TX, TY, PSTM_F32)                           \
                         : p->dtype == PSTM_BF16 ? PSTM_GRAD(TX, TY, PSTM_BF16)   \
                                                 : PSTM_GRAD(TX, TY, PSTM_F16))
-  if (L.tile_x == 4 && L.tile_y == 4) rc = PSTM_GRAD3(4, 4);
-  else if (L.tile_x == 16 && L.tile_y == 1) rc = PSTM_GRAD3(16, 1);
-  else if (L.tile_x == 1 && L.tile_y == 16) rc = PSTM_GRAD3(1, 16);
-  else rc = set_error(PSTM_ERR_INVALID, "unsupported tile shape %dx%d", L.tile_x, L.tile_y);
+  if (a.tile_x == 4 && a.tile_y == 4) rc = PSTM_GRAD3(4, 4);
+  else if (a.tile_x == 4 && a.tile_y == 2) rc = PSTM_GRAD3(4, 2);
+  else if (a.tile_x == 8 && a.tile_y == 1) rc = PSTM_GRAD3(8, 1);
+  else if (a.tile_x == 1 && a.tile_y == 8) rc = PSTM_GRAD3(1, 8);
+  else rc = set_error(PSTM_ERR_INVALID, "unsupported tile shape %dx%d", a.tile_x, a.tile_y);
 #undef PSTM_GRAD3
 #undef PSTM_GRAD
   return rc != PSTM_OK ? rc : check_launch("vel_grad");

@@ -632,12 +632,14 @@ int launch_forward_impl(const pstm_params_t* p, const PlanLayout& L, const TileA
              : PSTM_FWD(TX, TY, true, false, GENV))     \
       : (vti ? PSTM_FWD(TX, TY, false, true, GENV)      \
              : PSTM_FWD(TX, TY, false, false, GENV)))
-  if (L.tile_x == 4 && L.tile_y == 4) return gen ? PSTM_FWD4(4, 4, true) : PSTM_FWD4(4, 4, false);
-  if (L.tile_x == 16 && L.tile_y == 1) return PSTM_FWD4(16, 1, true);
-  if (L.tile_x == 1 && L.tile_y == 16) return PSTM_FWD4(1, 16, true);
+  if (a.tile_x == 4 && a.tile_y == 4 && !aa && !vti)
+    return gen ? PSTM_FWD(4, 4, false, false, true) : PSTM_FWD(4, 4, false, false, false);
+  if (a.tile_x == 4 && a.tile_y == 2) return gen ? PSTM_FWD4(4, 2, true) : PSTM_FWD4(4, 2, false);
+  if (a.tile_x == 8 && a.tile_y == 1) return PSTM_FWD4(8, 1, true);
+  if (a.tile_x == 1 && a.tile_y == 8) return PSTM_FWD4(1, 8, true);
 #undef PSTM_FWD4
 #undef PSTM_FWD
-  return set_error(PSTM_ERR_INVALID, "unsupported tile shape %dx%d", L.tile_x, L.tile_y);
+  return set_error(PSTM_ERR_INVALID, "unsupported tile shape %dx%d", a.tile_x, a.tile_y);
 }
 
 }  // namespace pstm

@@ -98,6 +98,18 @@ class PstmPlan:
         _need_cuda(vel, "vel", torch.float32, self.image_shape)
         return VEL_3D
 
+    def _common_layout(self, vel: torch.Tensor, eta: Optional[torch.Tensor]):
+        """The kernels read v_rms and eta in one layout (pstm_params_t.vel_mode):
+        a 1D field is broadcast to the image grid when the other is 3D."""
+        if eta is None or vel.dim() == eta.dim():
+            return vel, eta
+        shape = self.image_shape
+        if vel.dim() == 1:
+            vel = vel[None, None, :].expand(shape).contiguous()
+        else:
+            eta = eta[None, None, :].expand(shape).contiguous()
+        return vel, eta
+
     def _call_params(self, flags: Optional[int], dtype_code: int, vel_mode: int) -> PstmParams:
         p = self.params
         return p.replace(flags=p.flags if flags is None else flags, dtype=dtype_code, vel_mode=vel_mode)
@@ -120,6 +132,8 @@ class PstmPlan:
         if traces.dtype not in _DTYPE_CODE:
             raise ValueError(f"unsupported trace dtype {traces.dtype}")
         _need_cuda(traces, "traces", traces.dtype, (p.n_traces, p.n_samples))
+        flags_eff = p.flags if flags is None else flags
+        vel, eta = self._common_layout(vel, eta if flags_eff & FLAG_VTI else None)
         vm = self._vel_mode(vel)
         cp = self._call_params(flags, _DTYPE_CODE[traces.dtype], vm)
         if scale is not None:
@@ -148,6 +162,8 @@ class PstmPlan:
         """Demigrate ``image`` into FP32 traces (accumulates into ``out``)."""
         p = self.params
         _need_cuda(image, "image", torch.float32, self.image_shape)
+        flags_eff = p.flags if flags is None else flags
+        vel, eta = self._common_layout(vel, eta if flags_eff & FLAG_VTI else None)
         vm = self._vel_mode(vel)
         cp = self._call_params(flags, PSTM_F32, vm)
         if cp.flags & FLAG_VTI:

@@ -342,7 +342,9 @@ def config5_vti(seed: int = 5, n_bins: int = 200, n_classes: int = 60, class_min
     eta_d = eta.cpu()[ii, jj, kk].double()
     tr = synthesize(src.to(device), rcv.to(device), n_samples, dt, 30.0, diff, None, None, vfield=vf,
                     eta_diff=eta_d)
-    return SyntheticSurvey("cfg5-vti", p, src.to(device), rcv.to(device), tr, vf.on_grid(p, device), eta=eta,
+    vel = vf.on_grid(p, device)  # laterally constant: broadcast to eta's layout once
+    vel = vel[None, None, :].expand(n_img, n_img, n_samples).contiguous()
+    return SyntheticSurvey("cfg5-vti", p, src.to(device), rcv.to(device), tr, vel, eta=eta,
                            class_sizes=sizes, info=dict(diffractors=diff))
 
 

@@ -1,0 +1,223 @@
+"""Model / configuration layer (CPU).  Known answers restated from the
+reference's tests:
+  tests/test_config_adapter.py      wizard mapping, counts, validation
+  tests/test_velocity_model.py      constant / gradient v_rms
+  tests/test_anisotropy_model.py    eta formula, 1D interpolation
+  tests/test_kirchhoff_kernel.py:386-406  fold normalisation
+"""
+import json
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from seisproc_amd.abi_types import FLAG_AA, FLAG_KAHAN, FLAG_VTI, WEIGHT_FULL, WEIGHT_SPREADING
+from seisproc_amd.models import (AnisotropyModel, MigrationConfig, MigrationJobConfig, OutputGrid, SeismicData,
+                                 VelocityModel, WeightMode)
+from seisproc_amd.models.anisotropy_model import create_eta_field
+from seisproc_amd.models.velocity_model import (BSplineVelocity, create_linear_gradient_velocity,
+                                                interval_to_rms_velocity, rms_to_interval_velocity)
+from seisproc_amd.processors.migration.base_migrator import MigrationResult
+from seisproc_amd.processors.migration.config_adapter import ConfigAdapter
+from seisproc_amd.processors.migration.kirchhoff_kernel import normalize_by_fold
+from seisproc_amd.processors.migration import velocity_model as engine_vm
+
+WIZARD = {"name": "Test Migration", "x_origin": 1000.0, "y_origin": 2000.0, "output_bin_il": 25.0,
+          "output_bin_xl": 25.0, "grid_azimuth_deg": 45.0, "inline_min": 1, "inline_max": 100, "inline_step": 1,
+          "xline_min": 1, "xline_max": 100, "xline_step": 1, "time_min_ms": 0, "time_max_ms": 4000,
+          "dt_ms": 4.0, "velocity_type": "constant", "velocity_v0": 3000.0, "max_aperture_m": 5000.0,
+          "max_angle_deg": 60.0}
+
+
+# ---------------------------------------------------------------- ConfigAdapter
+def test_adapter_extracts_wizard_values():
+    p = ConfigAdapter(WIZARD).params
+    assert (p.origin_x, p.origin_y, p.il_spacing, p.xl_spacing, p.azimuth_deg) == (1000.0, 2000.0, 25.0, 25.0, 45.0)
+    assert (p.n_il, p.n_xl, p.dt_ms, p.t_min_ms, p.n_samples) == (100, 100, 4.0, 0.0, 1001)
+    assert (p.velocity_mps, p.max_aperture_m, p.max_angle_deg) == (3000.0, 5000.0, 60.0)
+
+
+def test_adapter_counts_with_steps_and_bin_fallbacks():
+    p = ConfigAdapter({"inline_min": 1, "inline_max": 199, "inline_step": 2, "xline_min": 1, "xline_max": 301,
+                       "xline_step": 3}).params
+    assert (p.n_il, p.n_xl) == (100, 101)
+    assert ConfigAdapter({"input_bin_il": 50.0, "input_bin_xl": 50.0}).params.il_spacing == 50.0
+    assert ConfigAdapter({}).params.xl_spacing == 25.0
+
+
+def test_adapter_engine_params_and_validation():
+    ad = ConfigAdapter(dict(WIZARD, output_bin_il=50.0, output_bin_xl=50.0, max_aperture_m=3000.0))
+    tr = np.zeros((10, 3), np.float32)
+    sx = np.arange(3.0)
+    kw = ad.get_engine_params(tr, sx, sx, sx, sx, normalize=False, min_fold=3)
+    assert kw["il_spacing"] == 50.0 and kw["n_times"] == 1001 and kw["normalize"] is False and kw["min_fold"] == 3
+    assert np.array_equal(kw["traces"], tr)
+    ok, msg = ConfigAdapter({k: v for k, v in WIZARD.items() if k != "grid_azimuth_deg"}).validate()
+    assert ok, msg
+    ok, msg = ConfigAdapter({"velocity_v0": 0.0}).validate()
+    assert not ok and "velocity" in msg.lower()
+    ok, msg = ConfigAdapter({"velocity_v0": 500.0}).validate()
+    assert not ok and "unusual" in msg.lower()
+    ok, msg = ConfigAdapter({"inline_min": 100, "inline_max": 10}).validate()
+    assert not ok and "inline" in msg.lower()
+    ok, msg = ConfigAdapter({"inline_max": 100000, "xline_max": 100000}).validate()
+    assert not ok and "large" in msg.lower()
+
+
+# ---------------------------------------------------------------- MigrationConfig
+def test_migration_config_validation_and_roundtrip():
+    with pytest.raises(ValueError, match="max_angle_deg"):
+        MigrationConfig(max_angle_deg=95.0)
+    with pytest.raises(ValueError, match="min_offset_m"):
+        MigrationConfig(min_offset_m=200.0, max_offset_m=100.0)
+    with pytest.raises(ValueError, match="n_time"):
+        OutputGrid(n_time=0)
+    c = MigrationConfig(weight_mode=WeightMode.FULL, max_aperture_m=1234.0)
+    c2 = MigrationConfig.from_dict(json.loads(json.dumps(c.to_dict())))
+    assert c2.to_dict() == c.to_dict()
+
+
+def test_config_to_kernel_params():
+    g = OutputGrid(n_time=501, n_inline=10, n_xline=20, dt=0.002, d_inline=12.5, d_xline=25.0, t0=0.1,
+                   x_origin=5e5, y_origin=6.7e6, inline_azimuth=30.0)
+    c = MigrationConfig(output_grid=g, weight_mode=WeightMode.FULL, antialias_enabled=True, use_anisotropy=True)
+    p = c.to_pstm_params(100, 1001, 0.004)
+    assert (p.n_il, p.n_xl, p.n_tau, p.tau0, p.dtau, p.dt) == (10, 20, 501, 0.1, 0.002, 0.004)
+    assert p.weight_mode == WEIGHT_FULL and p.flags & FLAG_AA and p.flags & FLAG_VTI and p.flags & FLAG_KAHAN
+    # the reference kernels ignore the azimuth: x = x0 + il d_il, y = y0 + xl d_xl
+    assert (p.il_dx, p.il_dy, p.xl_dx, p.xl_dy) == (12.5, 0.0, 0.0, 25.0)
+    # honoring the azimuth reproduces OutputGrid.get_coordinates
+    q = c.to_pstm_params(100, 1001, 0.004, honor_azimuth=True)
+    for i, j in ((3, 7), (9, 0), (0, 19)):
+        x, y = g.get_coordinates(i, j)
+        assert abs(q.origin_x + i * q.il_dx + j * q.xl_dx - x) < 1e-6
+        assert abs(q.origin_y + i * q.il_dy + j * q.xl_dy - y) < 1e-6
+    assert MigrationConfig().to_pstm_params(1, 10, 0.002).weight_mode == WEIGHT_SPREADING
+
+
+# ---------------------------------------------------------------- velocity
+def test_engine_velocity_model_known_answers():
+    m = engine_vm.VelocityModel("constant", 2500.0)
+    assert m.rms_velocity_at_time(1000.0) == 2500.0
+    assert np.allclose(m.rms_velocity_at_time(np.array([0.0, 500.0])), 2500.0)
+    g = engine_vm.VelocityModel("gradient", 2000.0, gradient=0.5)
+    assert g.interval_velocity(0.0) == 2000.0 and g.interval_velocity(1000.0) == 2500.0
+    assert g.rms_velocity_at_time(0.0) == 2000.0
+    v = [g.rms_velocity_at_time(t) for t in (500.0, 1000.0, 2000.0)]
+    assert v[0] < v[1] < v[2]
+    assert 1500.0 < engine_vm.VelocityModel("gradient", 1500.0, gradient=0.5).rms_velocity_at_time(2000.0) < 2500.0
+    t = torch.tensor([0.0, 500.0, 1000.0, 2000.0])
+    vt = g.rms_velocity_at_time(t)
+    assert isinstance(vt, torch.Tensor) and torch.isclose(vt[0], torch.tensor(2000.0)) and bool((vt[1:] > vt[:-1]).all())
+    # closed form at t = 1 s two-way: z = v0 t + k v0 t^2 / 2 with t = 0.5 s one-way
+    z = 2000.0 * 0.5 + 0.5 * 0.5 * 2000.0 * 0.25
+    assert abs(g.rms_velocity_at_time(1000.0) - math.sqrt(2000.0**2 + 2000.0 * 0.5 * z + (0.5 * z) ** 2 / 3)) < 1e-6
+
+
+def test_velocity_model_types_and_dix_roundtrip():
+    assert VelocityModel(2000.0).is_constant
+    m = create_linear_gradient_velocity(1500.0, 600.0, 3.0, 0.002)
+    assert abs(m.rms_velocity_at_time(np.array([1.0]))[0] - 2100.0) < 1e-3
+    t = np.linspace(0.0, 3.0, 31)
+    vi = 1500.0 + 400.0 * t
+    _, vr = interval_to_rms_velocity(t, vi)
+    _, vi2 = rms_to_interval_velocity(t, vr)
+    np.testing.assert_allclose(vi2[1:], vi[1:], rtol=2e-3)
+    v3 = VelocityModel(data=np.full((3, 4, 5), 2000.0, np.float32), z_axis=np.linspace(0, 4, 3),
+                       x_axis=np.arange(4.0) * 100, y_axis=np.arange(5.0) * 100)
+    assert np.allclose(v3.get_velocity_at(1.0, 150.0, 220.0), 2000.0)
+    d = VelocityModel.from_dict(json.loads(json.dumps(m.to_dict())))
+    np.testing.assert_allclose(d.data, m.data)
+
+
+def test_bspline_velocity_partition_of_unity_and_fit():
+    b = BSplineVelocity(20, 18, 60, shape=(6, 5, 8), device="cpu")
+    coef = torch.full((6, 5, 8), 2500.0)
+    assert torch.allclose(b.evaluate(coef), torch.full((20, 18, 60), 2500.0), atol=1e-3)
+    ii = torch.linspace(0, 1, 20)[:, None, None]
+    kk = torch.linspace(0, 1, 60)[None, None, :]
+    v = (1800.0 + 300.0 * ii + 600.0 * kk).expand(20, 18, 60).contiguous()
+    fit = b.evaluate(b.fit(v))
+    assert float((fit - v).abs().max()) < 1.0  # cubic B-splines reproduce linear fields
+
+
+# ---------------------------------------------------------------- anisotropy
+def test_eta_formula_and_interpolation():
+    assert abs(AnisotropyModel.compute_eta(0.25, 0.1) - (0.25 - 0.1) / 1.2) < 1e-10
+    assert AnisotropyModel.compute_eta(0.0, 0.0) == 0.0
+    assert abs(AnisotropyModel.compute_eta(0.2, 0.2)) < 1e-10
+    m = AnisotropyModel(epsilon=np.array([0.1, 0.2, 0.3]), delta=np.array([0.05, 0.1, 0.15]),
+                        z_axis=np.array([0.0, 1.0, 2.0]))
+    eps, dl, eta = m.get_parameters_at(0.5)
+    assert abs(eps - 0.15) < 1e-6 and abs(dl - 0.075) < 1e-6
+    with pytest.raises(ValueError):
+        AnisotropyModel(0.1, -0.6)
+
+
+def test_eta_field_on_grid():
+    from seisproc_amd.abi_types import PstmParams
+    p = PstmParams.from_dict(dict(n_il=3, n_xl=4, n_tau=5, dtau=0.5, il_dx=10.0, xl_dy=10.0))
+    eta = np.random.default_rng(0).uniform(0, 0.15, (3, 4, 5))
+    m = create_eta_field(eta, np.arange(3) * 10.0, np.arange(4) * 10.0, np.arange(5) * 0.5)
+    np.testing.assert_allclose(m.eta_on_grid(p, device="cpu").numpy(), eta, atol=1e-6)
+    c = AnisotropyModel(0.1, 0.0)
+    assert torch.allclose(c.eta_on_grid(p, device="cpu"), torch.full((5,), 0.1))
+
+
+# ---------------------------------------------------------------- normalisation / results
+def test_normalize_by_fold_known_answers():
+    # tests/test_kirchhoff_kernel.py:386-406
+    img = torch.tensor([[[10.0, 20.0], [30.0, 0.0]]])
+    fold = torch.tensor([[[2.0, 4.0], [3.0, 0.0]]])
+    out = normalize_by_fold(img, fold, 1)
+    assert out.tolist() == [[[5.0, 5.0], [10.0, 0.0]]]
+    out = normalize_by_fold(torch.tensor([[[10.0, 20.0]]]), torch.tensor([[[1.0, 0.5]]]), 1)
+    assert out.tolist() == [[[10.0, 20.0]]]
+
+
+def test_migration_result_fold_normalization():
+    r = MigrationResult(np.array([[[10.0, 20.0, 7.0]]]), np.array([[[2, 4, 1]]]))
+    n = r.apply_fold_normalization(min_fold=2)
+    assert n.image.tolist() == [[[5.0, 5.0, 0.0]]] and n.metadata["fold_normalized"]
+    assert r.shape == (1, 1, 3) and r.get_statistics()["fold_max"] == 4
+
+
+# ---------------------------------------------------------------- job config / data / processor params
+def test_job_config_roundtrip(tmp_path):
+    j = MigrationJobConfig(name="j", input_file=str(tmp_path / "in.zarr"), output_directory=str(tmp_path / "out"),
+                           inline_max=199, inline_step=2, time_max_ms=2000, dt_ms=2.0, binning_preset="land_3d")
+    g = j.get_output_grid()
+    assert (g.n_inline, g.n_xline, g.n_time, g.dt) == (100, 100, 1001, 0.002)
+    f = tmp_path / "job.json"
+    j.save(str(f))
+    j2 = MigrationJobConfig.load(str(f))
+    assert j2.to_dict() == j.to_dict()
+    errs = j.validate()
+    assert any("does not exist" in e for e in errs)
+    assert j.get_migration_config().weight_mode is WeightMode.SPREADING
+    vf = tmp_path / "v.txt"
+    np.savetxt(vf, np.array([[0.0, 1500.0], [2.0, 2700.0]]))
+    j3 = MigrationJobConfig(name="v", input_file="x", output_directory=str(tmp_path), velocity_file=str(vf))
+    assert j3.get_velocity_model().v0 == 1500.0
+
+
+def test_seismic_data_contract():
+    d = SeismicData(traces=np.zeros((11, 3), np.int16), sample_rate=2.0)
+    assert d.traces.dtype == np.float32 and d.n_samples == 11 and d.duration == 20.0 and d.nyquist_freq == 250.0
+    with pytest.raises(ValueError):
+        SeismicData(traces=np.zeros(5), sample_rate=2.0)
+
+
+def test_pstm_processor_params_roundtrip_without_gpu():
+    from seisproc_amd.processors import BaseProcessor, PROCESSOR_REGISTRY
+    cfg = MigrationConfig(output_grid=OutputGrid(n_time=11, n_inline=2, n_xline=3))
+    proc = PROCESSOR_REGISTRY["PSTMProcessor"](config=cfg, velocity=2000.0, trace_dtype="bf16")
+    d = json.loads(json.dumps(proc.to_dict()))
+    again = BaseProcessor.from_dict(d)
+    assert again.config.to_dict() == cfg.to_dict() and again.params["trace_dtype"] == "bf16"
+    assert "HIP" in again.get_description()
+    with pytest.raises(ValueError):
+        PROCESSOR_REGISTRY["PSTMProcessor"](config=cfg, velocity=2000.0, trace_dtype="int8")
+    with pytest.raises(ValueError):
+        BaseProcessor.from_dict({"module": "seisproc_amd.models", "class_name": "SeismicData", "params": {}})
