@@ -69,8 +69,8 @@ def migrate_kirchhoff_full(traces: torch.Tensor, source_x: torch.Tensor, source_
     z = depth_axis.detach().double().cpu().numpy()
     tau = 2.0 * z / float(velocity)
     n_tau = len(tau)
-    dtau = float(tau[1] - tau[0]) if n_tau > 1 else 1.0
-    if n_tau > 2 and not np.allclose(np.diff(tau), dtau, rtol=1e-5, atol=1e-9):
+    dtau = float(tau[-1] - tau[0]) / (n_tau - 1) if n_tau > 1 else 1.0
+    if n_tau > 2 and np.abs(tau - (tau[0] + dtau * np.arange(n_tau))).max() > 1e-3 * abs(dtau):
         raise ValueError("depth_axis must be regular in two-way time (z = v t / 2)")
     x0, y0, il, xl = _grid_from_points(output_x, output_y, n_il, n_xl)
     p = PstmParams.from_dict(dict(
