@@ -70,12 +70,12 @@ static int validate(const pstm_params_t* p) {
     if (!(p->aa_scale >= 0)) return set_error(PSTM_ERR_INVALID, "aa_scale must be >= 0");
     pad = (int)ceil(p->aa_max) + 2;
   }
-  // A staged window never exceeds n_samples + 2 (pad + margin) + 1 samples;
-  // with the antialias filter each sample is held in FP64 (two floats).
-  const int64_t elem = (p->flags & PSTM_FLAG_AA) ? 2 : 1;
-  if (elem * ((int64_t)p->n_samples + 2 * (pad + kWindowMargin) + 2) > kSampleCap)
-    return set_error(PSTM_ERR_UNSUPPORTED, "n_samples %d too long for the LDS sample buffer (%d floats%s)",
-                     p->n_samples, kSampleCap, elem == 2 ? ", FP64 windows with antialias" : "");
+  // A staged window never exceeds n_samples + 2 (pad + margin) + 1 samples.
+  const bool aa = (p->flags & PSTM_FLAG_AA) != 0;
+  const int64_t cap = min_cap_elems(aa);
+  if ((int64_t)p->n_samples + 2 * (pad + kWindowMargin) + 2 > cap)
+    return set_error(PSTM_ERR_UNSUPPORTED, "n_samples %d too long for the LDS sample buffer (%lld samples%s)",
+                     p->n_samples, (long long)cap, aa ? " with antialias" : "");
   if ((int64_t)p->n_il * p->n_xl * (int64_t)p->n_tau > ((int64_t)1 << 40))
     return set_error(PSTM_ERR_INVALID, "image too large");
   return PSTM_OK;

@@ -16,10 +16,62 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kTileTau = kThreads;
 
-// LDS capacities of the tile kernels.
-constexpr int kSampleCap = 6144;  // staged trace samples (floats) per sub-batch
 constexpr int kMaxBatch = 64;     // traces per sub-batch
 constexpr int kQueueCap = 320;    // candidate queue entries (>= kMaxBatch - 1 + kThreads)
+
+// LDS budget of one tile workgroup (160 KB per CU, allocated in 512-byte
+// granules): the plain kernels run 3 workgroups per CU, the antialias
+// kernels (more registers) 2.  Three workgroups of 54272 bytes (159 KB)
+// were measured to fit only two per CU; 53760 bytes (157.5 KB for three)
+// fit three.
+constexpr int kLdsBudget = 53760;
+constexpr int kLdsBudgetAA = 80896;
+static_assert(kLdsBudget % 512 == 0 && kLdsBudgetAA % 512 == 0, "512-byte LDS granules");
+
+// Tile-kernel LDS besides the staged sample windows.
+struct TileShared {
+  int32_t q_j[kQueueCap];       // candidate queue: sorted trace index
+  int32_t q_w0[kQueueCap];      // first staged sample
+  int32_t q_len[kQueueCap];     // staged window length
+  int32_t b_base[kMaxBatch];    // window-element index of sample 0 of the trace (off - w0)
+  int32_t b_off[kMaxBatch];     // window-element offset of the window
+  int32_t scan[kWaves];
+  int32_t queue_n;
+  int32_t nb;
+  float hwave[kWaves];          // max hlim^2 of each wave (over its columns and times)
+  float qmin[kWaves], qmax[kWaves], tqa[kWaves], tqb[kWaves];
+  float emax;                   // max eta over the tile (VTI)
+};
+
+// Per sub-batch (trace, column) quantities of a tile of NC columns.
+template <int NC, bool AA>
+struct PairShared {
+  float2 h2[kMaxBatch][NC];     // squared source / receiver distances (m^2)
+  float hm[kMaxBatch][NC];      // max of the two
+  int32_t act[kMaxBatch];       // bit w: the trace reaches some time of wave w
+  float hcol[kWaves][NC];       // per wave and column: max hlim^2 over the wave's times
+  float4 g[AA ? kMaxBatch : 1][AA ? NC : 1];  // antialias: scaled (s - x, r - x) components
+};
+
+// Staged-window capacity in floats and in window elements.  A window
+// element is one FP32 sample, or with the antialias filter the FP64 double
+// integral plus the FP32 single integral (three floats).
+template <int NC, bool AA>
+constexpr int sample_cap_floats() {
+  return (((AA ? kLdsBudgetAA : kLdsBudget) - (int)sizeof(TileShared) - (int)sizeof(PairShared<NC, AA>)) / 4) &
+         ~63;
+}
+template <int NC, bool AA>
+constexpr int sample_cap_elems() {
+  return AA ? sample_cap_floats<NC, AA>() / 3 : sample_cap_floats<NC, AA>();
+}
+// Smallest capacity over the launched tile shapes (8 and 16 columns;
+// antialias only with 8).
+constexpr int min_cap_elems(bool aa) {
+  return aa ? sample_cap_elems<8, true>()
+            : (sample_cap_elems<8, false>() < sample_cap_elems<16, false>() ? sample_cap_elems<8, false>()
+                                                                             : sample_cap_elems<16, false>());
+}
 
 // Margin (samples) added on each side of a staged window beyond the
 // rigorous traveltime bounds (absorbs FP32 rounding of the bounds).

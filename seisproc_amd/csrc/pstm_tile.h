@@ -50,37 +50,93 @@ __device__ __forceinline__ float uniform_f(float v) {
 }
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Inclusive prefix sum over the 64 lanes of a wave.
+// DPP lane moves (gfx9 encodings).  A lane whose source is outside its row,
+// or whose row is masked off, receives 0.
+enum : int {
+  kDppQuadXor1 = 0xB1,  // quad_perm [1, 0, 3, 2]
+  kDppQuadXor2 = 0x4E,  // quad_perm [2, 3, 0, 1]
+  kDppRowShr1 = 0x111,
+  kDppRowShr2 = 0x112,
+  kDppRowShr4 = 0x114,
+  kDppRowShr8 = 0x118,
+  kDppRowBcast15 = 0x142,  // lane 15 of each row to the next row
+  kDppRowBcast31 = 0x143,  // lane 31 to rows 2 and 3
+};
+
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ int dpp_move(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(dpp_move<CTRL, ROWS>(__float_as_int(v)));
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_move<CTRL, ROWS>((int)(b & 0xffffffffll)), hi = dpp_move<CTRL, ROWS>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave (row-local doubling
+// steps, then the row carries).
 template <typename T>
 __device__ __forceinline__ T wave_scan(T v, int lane) {
-  for (int off = 1; off < 64; off <<= 1) {
-    T o = __shfl_up(v, off);
-    if (lane >= off) v += o;
-  }
+  (void)lane;
+  v += dpp_move<kDppRowShr1>(v);
+  v += dpp_move<kDppRowShr2>(v);
+  v += dpp_move<kDppRowShr4>(v);
+  v += dpp_move<kDppRowShr8>(v);
+  v += dpp_move<kDppRowBcast15, 0xa>(v);
+  v += dpp_move<kDppRowBcast31, 0xc>(v);
   return v;
 }
 
-// Causal then anticausal running sum of one staged window (one wave).  With
-// the antialias filter the window holds FP64 values: the double integral of
-// an FP32 trace is exact in FP64, and its second difference is what the
-// triangle taps read.
-template <typename T>
-__device__ __forceinline__ void integrate_window(T* w, int len, int lane) {
-  T carry = 0;
-  for (int s0 = 0; s0 < len; s0 += 64) {
-    const int s = s0 + lane;
-    T v = s < len ? w[s] : T(0);
-    v = wave_scan(v, lane) + carry;
-    if (s < len) w[s] = v;
-    carry = __shfl(v, 63);
+// Value of lane 63.
+__device__ __forceinline__ double lane63(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// OR over each aligned group of NC (4, 8 or 16) lanes; the result is valid in
+// the group's last lane.
+template <int NC>
+__device__ __forceinline__ int group_or(int v) {
+  static_assert(NC == 4 || NC == 8 || NC == 16, "group of 4, 8 or 16 lanes");
+  v |= dpp_move<kDppQuadXor1>(v);
+  v |= dpp_move<kDppQuadXor2>(v);
+  if constexpr (NC >= 8) v |= dpp_move<kDppRowShr4>(v);
+  if constexpr (NC >= 16) v |= dpp_move<kDppRowShr8>(v);
+  return v;
+}
+
+// Causal then anticausal running sum of one staged FP64 window (one wave):
+// A[s] <- sum_{k >= s} sum_{j <= k} A[j].  Each lane sums a contiguous
+// segment serially; one wave scan per pass joins the segments.  With C
+// non-null the causal sums are also stored, rounded to FP32, in C.  The
+// double integral of an FP32 trace is held in FP64 because the triangle taps
+// take its second difference; the same symmetric operator is its own
+// transpose (adjoint windows).
+__device__ __forceinline__ void integrate_window(double* A, float* C, int len, int lane) {
+  const int seg = ((len + 63) >> 6) | 1;  // odd stride: conflict-free segment reads
+  const int s0 = min(lane * seg, len), s1 = min(s0 + seg, len);
+  double t = 0.0;
+  for (int s = s0; s < s1; ++s) t += A[s];
+  double run = wave_scan(t, lane) - t;  // sum of the segments before this lane's
+  for (int s = s0; s < s1; ++s) {
+    run += A[s];
+    A[s] = run;
+    if (C) C[s] = (float)run;
   }
-  carry = 0;
-  for (int s0 = 0; s0 < len; s0 += 64) {
-    const int s = len - 1 - (s0 + lane);
-    T v = s >= 0 ? w[s] : T(0);
-    v = wave_scan(v, lane) + carry;
-    if (s >= 0) w[s] = v;
-    carry = __shfl(v, 63);
+  t = 0.0;
+  for (int s = s0; s < s1; ++s) t += A[s];
+  const double incl = wave_scan(t, lane);
+  run = lane63(incl) - incl;  // sum of the segments after this lane's
+  for (int s = s1 - 1; s >= s0; --s) {
+    run += A[s];
+    A[s] = run;
   }
 }
 
@@ -113,39 +169,33 @@ __device__ __forceinline__ float taper_of(float hm2, float hl2, float tw) {
   return 0.5f * (1.0f + __cosf(3.14159265358979f * fminf(x, 1.0f)));
 }
 
-struct alignas(16) TileShared {
-  alignas(16) float samp[kSampleCap];  // staged windows: FP32, or FP64 with antialias
-  int32_t q_j[kQueueCap];
-  int32_t q_w0[kQueueCap];
-  int32_t q_len[kQueueCap];
-  int32_t b_base[kMaxBatch];   // window-element index of sample 0 of the trace (off - w0)
-  int32_t b_off[kMaxBatch];    // window-element offset of the window
-  int32_t scan[kWaves];
-  int32_t queue_n;
-  int32_t nb;
-  float hwave[kWaves];          // max hlim^2 of each wave (over its columns and times)
-  float qmin[kWaves], qmax[kWaves], tqa[kWaves], tqb[kWaves];
-  float emax;                   // max eta over the tile (VTI)
-};
-
-template <int NC, bool AA>
-struct alignas(16) PairShared {
-  float2 h2[kMaxBatch][NC];
-  float4 g[AA ? kMaxBatch : 1][AA ? NC : 1];
-};
-
 // GEN = false: FULL weight, no taper, clamped record end (compile-time);
 // GEN = true: weight mode, taper and zero-outside read from the arguments.
+//
+// Inside the kernel traveltimes are in samples: with qd = 1/(v dt)^2 and
+// tqs = tau^2/(4 dt^2) a leg is t/dt = sqrt(h^2 qd + tqs) and rs = dt/t.
+// Each weight is a per-lane, per-column constant cw, applied once per
+// sub-batch, times a power of prs = rs rr = dt^2/(t_s t_r):
+//   FULL       tau^2/4 q (t_s t_r)^-2 = cw prs^2,  cw = tau^2/4 q / dt^4
+//   SPREADING  q (t_s t_r)^-1         = cw prs,    cw = q / dt^2
+//   OBLIQUITY  tau^2/4 (t_s t_r)^-1   = cw prs,    cw = tau^2/4 / dt^2
+//   NONE       1                      = cw,        cw = 1
 template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN>
 __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
   constexpr int NC = TX * TY;
-  using WT = typename std::conditional<AA, double, float>::type;  // staged window element
-  constexpr int WF = (int)(sizeof(WT) / sizeof(float));           // floats per element
+  static_assert(NC == 4 || NC == 8 || NC == 16, "columns per tile: 4, 8 or 16");
+  constexpr int CAPF = sample_cap_floats<NC, AA>();  // staged-window floats
+  constexpr int CAPE = sample_cap_elems<NC, AA>();   // staged-window elements
   __shared__ TileShared S;
   __shared__ PairShared<NC, AA> P;
-  WT* const Wn = reinterpret_cast<WT*>(S.samp);
+  __shared__ __align__(16) float samp[CAPF];
+  // Staged windows: FP32 samples, or with antialias the FP64 double integral
+  // (Wa) and the FP32 single integral (Wc, forward only).
+  [[maybe_unused]] float* const Wf = samp;
+  [[maybe_unused]] double* const Wa = reinterpret_cast<double*>(samp);
+  [[maybe_unused]] float* const Wc = samp + 2 * CAPE;
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
   // ---- block -> (time block, tile): deepest time blocks first, each XCD a
   // contiguous spatial chunk of tiles.
@@ -163,10 +213,16 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
   const int kk = kvalid ? k : a.n_tau - 1;
   const float tau = a.tau0 + (float)kk * a.dtau;
   const float tq = 0.25f * tau * tau;
-  const float tqx = fmaxf(tq, 1e-12f);
+  const float idt2 = a.inv_dt * a.inv_dt;
+  const float tqs = fmaxf(tq, 1e-12f) * idt2;
+  [[maybe_unused]] const float vfloor = 1e-12f * idt2;
+  const int wmode = GEN ? a.weight_mode : (int)PSTM_WEIGHT_FULL;
+  [[maybe_unused]] const float aa_scale = AA ? a.aa_scale_over_dt / a.inv_dt : 0.f;
 
-  float q[NC], hl2[NC];
+  float qd[NC], hl2[NC], cw[NC];
   [[maybe_unused]] float k1[VTI ? NC : 1], m2[VTI ? NC : 1];
+  [[maybe_unused]] float sv[MODE == MODE_GRAD ? NC : 1];  // 1/v
+  [[maybe_unused]] float qa[AA ? NC : 1];                 // qd * aa_scale
   float emax_l = 0.f;
   float qmin_l = 3.0e38f, qmax_l = 0.f, hmax_l = -1.f;
   unsigned cvalid = 0;  // bit c: column c of the tile is inside the image
@@ -181,7 +237,14 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
     const float v = a.vel[vi];
     const bool ok = kvalid && cv && v > 0.f;
     const float vv = ok ? v : 1.f;
-    q[c] = 1.0f / (vv * vv);
+    const float q = 1.0f / (vv * vv);
+    qd[c] = q * idt2;
+    if (wmode == PSTM_WEIGHT_FULL) cw[c] = (tq * q) * (idt2 * idt2);
+    else if (wmode == PSTM_WEIGHT_SPREADING) cw[c] = q * idt2;
+    else if (wmode == PSTM_WEIGHT_OBLIQUITY) cw[c] = tq * idt2;
+    else cw[c] = 1.f;
+    if constexpr (MODE == MODE_GRAD) sv[c] = 1.0f / vv;
+    if constexpr (AA) qa[c] = qd[c] * aa_scale;
     const float z = 0.5f * vv * tau;
     float ap = a.aperture;
     if (a.flags & PSTM_FLAG_TIME_APERTURE) ap = fminf(ap, z * a.tan_th);
@@ -194,15 +257,17 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
       emax_l = fmaxf(emax_l, e);
     }
     if (ok) {
-      qmin_l = fminf(qmin_l, q[c]);
-      qmax_l = fmaxf(qmax_l, q[c]);
+      qmin_l = fminf(qmin_l, q);
+      qmax_l = fmaxf(qmax_l, q);
     }
     hmax_l = fmaxf(hmax_l, hl2[c]);
   }
-  // wave-uniform per-column reach (max over the wave's times)
-  float hcol[NC];
+  // per-wave column reach (max over the wave's times)
 #pragma unroll
-  for (int c = 0; c < NC; ++c) hcol[c] = uniform_f(wave_max(hl2[c]));
+  for (int c = 0; c < NC; ++c) {
+    const float hc = wave_max(hl2[c]);
+    if (lane == 0) P.hcol[wave][c] = hc;
+  }
   {
     const float hw = wave_max(hmax_l), qn = wave_min(qmin_l), qx = wave_max(qmax_l);
     const float em = wave_max(emax_l);
@@ -244,11 +309,11 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const int il = tx * TX + c / TY, xl = ty * TY + c % TY;
-      m_in[c] = (kvalid && ((cvalid >> c) & 1u)) ? a.image_in[((int64_t)il * a.n_xl + xl) * a.n_tau + k] : 0.f;
+      m_in[c] = (kvalid && ((cvalid >> c) & 1u)) ? a.image_in[((int64_t)il * a.n_xl + xl) * a.n_tau + k] * cw[c]
+                                                 : 0.f;
     }
   }
 
-  const bool wm_full = GEN ? a.weight_mode == PSTM_WEIGHT_FULL : true;
   const bool use_taper = GEN ? (a.flags & PSTM_FLAG_TAPER) != 0 : false;
   const bool zero_out = GEN ? (a.flags & PSTM_FLAG_ZERO_OUTSIDE) != 0 : false;
 
@@ -280,12 +345,12 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
         if (wave == 0) {
           const int i = qs + lane;
           const bool in = lane < kMaxBatch && i < qn;
-          const int cost = in ? WF * S.q_len[i] : 0;
+          const int cost = in ? S.q_len[i] : 0;  // window elements
           const int incl = wave_scan(cost, lane);
-          const bool fit = in && incl <= kSampleCap;
+          const bool fit = in && incl <= CAPE;
           const uint64_t m = __ballot(fit);
           if (fit) {
-            const int off = (incl - cost) / WF;  // in window elements
+            const int off = incl - cost;
             S.b_off[lane] = off;
             S.b_base[lane] = off - S.q_w0[i];
           }
@@ -299,7 +364,10 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
           const int e = qs + b;
           const int w0 = S.q_w0[e], len = S.q_len[e], off = S.b_off[b];
           if constexpr (MODE == MODE_ADJ) {
-            for (int s = lane; s < len; s += 64) Wn[off + s] = WT(0);
+            for (int s = lane; s < len; s += 64) {
+              if constexpr (AA) Wa[off + s] = 0.0;
+              else Wf[off + s] = 0.f;
+            }
           } else {
             const int id = a.ids[S.q_j[e]];
             const float sc = a.scale ? a.scale[id] : 1.f;
@@ -308,132 +376,134 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
               const int src = w0 + s;
               float v = 0.f;
               if (src >= 0 && src < a.n_samples) v = load_sample<DT>(a.traces, row + src) * sc;
-              Wn[off + s] = (WT)v;
+              if constexpr (AA) Wa[off + s] = (double)v;
+              else Wf[off + s] = v;
             }
           }
         }
-        // -- (trace, column) distances
+        // -- (trace, column) distances and the waves each trace reaches.  The
+        // NC columns of a trace sit in NC consecutive lanes of one wave.
         for (int idx = tid; idx < nb * NC; idx += kThreads) {
           const int b = idx / NC, c = idx % NC;
           const float4 g = a.geom[S.q_j[qs + b]];
           const float X = (float)(tx * TX + c / TY) * a.d_il, Y = (float)(ty * TY + c % TY) * a.d_xl;
           const float dxs = X - g.x, dys = Y - g.y, dxr = X - g.z, dyr = Y - g.w;
-          P.h2[b][c] = make_float2(fmaf(dxs, dxs, dys * dys), fmaf(dxr, dxr, dyr * dyr));
-          if constexpr (AA) P.g[b][c] = make_float4(dxs, dys, dxr, dyr);
+          const float hs = fmaf(dxs, dxs, dys * dys), hr = fmaf(dxr, dxr, dyr * dyr);
+          const float hm = fmaxf(hs, hr);
+          P.h2[b][c] = make_float2(hs, hr);
+          P.hm[b][c] = hm;
+          if constexpr (AA) P.g[b][c] = make_float4(dxs * a.d_il, dys * a.d_xl, dxr * a.d_il, dyr * a.d_xl);
+          int bits = 0;
+#pragma unroll
+          for (int w = 0; w < kWaves; ++w) bits |= hm < P.hcol[w][c] ? (1 << w) : 0;
+          bits = group_or<NC>(bits);
+          if (c == NC - 1) P.act[b] = bits;
         }
         __syncthreads();
         if constexpr (AA && MODE != MODE_ADJ) {
           // double integration (causal, then anticausal) of each window
-          for (int b = wave; b < nb; b += kWaves) integrate_window(Wn + S.b_off[b], S.q_len[qs + b], lane);
+          for (int b = wave; b < nb; b += kWaves)
+            integrate_window(Wa + S.b_off[b], Wc + S.b_off[b], S.q_len[qs + b], lane);
           __syncthreads();
         }
 
-        // -- contributions
-        for (int b = 0; b < nb; ++b) {
+        // -- contributions of the traces that reach this wave's samples
+        const uint64_t todo = __ballot(lane < nb && ((P.act[lane] >> wave) & 1));
+        for (uint64_t m = todo; m != 0; m &= m - 1) {
+          const int b = uniform_i(__builtin_ctzll(m));
           const int base = uniform_i(S.b_base[b]);
-          float2 h2[NC];
-          bool any = false;
+          // window of the trace, indexed by sample
+          [[maybe_unused]] float* const Fb = Wf + base;
+          [[maybe_unused]] double* const Ab = Wa + base;
+          [[maybe_unused]] float* const Cb = Wc + base;
 #pragma unroll
           for (int c = 0; c < NC; ++c) {
-            h2[c] = P.h2[b][c];
-            any |= fmaxf(h2[c].x, h2[c].y) < hcol[c];
-          }
-          if (!any) continue;  // the trace reaches none of this wave's samples
-#pragma unroll
-          for (int c = 0; c < NC; ++c) {
-            const float hm = fmaxf(h2[c].x, h2[c].y);
-            const float xs = fmaf(h2[c].x, q[c], tqx), xr = fmaf(h2[c].y, q[c], tqx);
-            const float rs = __builtin_amdgcn_rsqf(xs), rr = __builtin_amdgcn_rsqf(xr);
-            float t;
-            if constexpr (VTI) {
-              const float ps = h2[c].x * q[c], pr = h2[c].y * q[c];
-              const float ds = fmaf(k1[c], ps, tqx), dr = fmaf(k1[c], pr, tqx);
-              const float vs = fmaf(m2[c] * ps, ps * __builtin_amdgcn_rcpf(ds), tqx + ps);
-              const float vr = fmaf(m2[c] * pr, pr * __builtin_amdgcn_rcpf(dr), tqx + pr);
-              t = __builtin_amdgcn_sqrtf(fmaxf(vs, 1e-12f)) + __builtin_amdgcn_sqrtf(fmaxf(vr, 1e-12f));
-            } else {
-              t = fmaf(xs, rs, xr * rr);
-            }
-            const float u_raw = fmaf(t, a.inv_dt, -a.t0_over_dt);
+            const float2 h2 = P.h2[b][c];
+            const float hm = P.hm[b][c];
             bool ok = hm < hl2[c];
+            const float xs = fmaf(h2.x, qd[c], tqs), xr = fmaf(h2.y, qd[c], tqs);
+            const float rs = __builtin_amdgcn_rsqf(xs), rr = __builtin_amdgcn_rsqf(xr);
+            float u_raw;
+            if constexpr (VTI) {
+              const float ps = h2.x * qd[c], pr = h2.y * qd[c];
+              const float ds = fmaf(k1[c], ps, tqs), dr = fmaf(k1[c], pr, tqs);
+              const float vs = fmaf(m2[c] * ps, ps * __builtin_amdgcn_rcpf(ds), tqs + ps);
+              const float vr = fmaf(m2[c] * pr, pr * __builtin_amdgcn_rcpf(dr), tqs + pr);
+              u_raw = __builtin_amdgcn_sqrtf(fmaxf(vs, vfloor)) + __builtin_amdgcn_sqrtf(fmaxf(vr, vfloor)) -
+                      a.t0_over_dt;
+            } else {
+              u_raw = fmaf(xs, rs, fmaf(xr, rr, -a.t0_over_dt));
+            }
             if (zero_out) ok = ok && (u_raw >= 0.f) && (u_raw <= a.u_last);
             const float u = __builtin_amdgcn_fmed3f(u_raw, 0.f, a.umax);
             const float prs = rs * rr;
-            float w;
-            if (wm_full) {
-              w = (tq * q[c]) * (prs * prs);
-            } else if (a.weight_mode == PSTM_WEIGHT_SPREADING) {
-              w = q[c] * prs;
-            } else if (a.weight_mode == PSTM_WEIGHT_OBLIQUITY) {
-              w = tq * prs;
-            } else {
-              w = 1.f;
+            float wd = prs * prs;
+            if constexpr (GEN) {
+              if (wmode != PSTM_WEIGHT_FULL) wd = wmode == PSTM_WEIGHT_NONE ? 1.f : prs;
+              if (use_taper) wd *= taper_of(hm, hl2[c], a.taper_width);
             }
-            if (use_taper) w *= taper_of(hm, hl2[c], a.taper_width);
-            float L = 1.f, iL2 = 1.f;
+            [[maybe_unused]] float L = 1.f, iL2 = 1.f;
             if constexpr (AA) {
               const float4 g = P.g[b][c];
               const float ga = fmaf(g.x, rs, g.z * rr), gb = fmaf(g.y, rs, g.w * rr);
-              L = q[c] * a.aa_scale_over_dt * fmaf(a.d_il, fabsf(ga), a.d_xl * fabsf(gb));
-              L = __builtin_amdgcn_fmed3f(L, 1.f, a.aa_max);
+              L = __builtin_amdgcn_fmed3f(qa[c] * (fabsf(ga) + fabsf(gb)), 1.f, a.aa_max);
               iL2 = __builtin_amdgcn_rcpf(L * L);
             }
             const int i0 = (int)u;  // u >= 0: truncation is floor
             const float f = __builtin_amdgcn_fractf(u);
             if constexpr (MODE == MODE_FWD || MODE == MODE_GRAD) {
-              float amp, a0 = 0.f, a1 = 0.f;
+              float amp;
+              [[maybe_unused]] float a0 = 0.f, a1 = 0.f;
               if constexpr (AA) {
-                // triangle taps on the FP64 double integral
+                // triangle taps: second difference of the double integral A,
+                // A(i + f) = A(i) - f C(i) with C the single integral
                 const float um = u - L, up = u + L;
-                const int im = (int)floorf(um), ip = (int)up;
-                const double fd = (double)f, fmd = (double)(um - (float)im), fpd = (double)(up - (float)ip);
-                const double A0 = Wn[base + i0], A1 = Wn[base + i0 + 1];
-                const double B0 = Wn[base + im], B1 = Wn[base + im + 1];
-                const double C0 = Wn[base + ip], C1 = Wn[base + ip + 1];
-                const double ac = fma(fd, A1 - A0, A0), am = fma(fmd, B1 - B0, B0), ap2 = fma(fpd, C1 - C0, C0);
-                amp = (float)(2.0 * ac - am - ap2) * iL2;
+                const float flm = floorf(um);
+                const int im = (int)flm, ip = (int)up;
+                const float fm = um - flm, fp = __builtin_amdgcn_fractf(up);
+                const double y2 = fma(2.0, Ab[i0], -Ab[im]) - Ab[ip];
+                const float yc = fmaf(-2.f * f, Cb[i0], fmaf(fm, Cb[im], fp * Cb[ip]));
+                amp = ((float)y2 + yc) * iL2;
               } else {
-                a0 = Wn[base + i0];
-                a1 = Wn[base + i0 + 1];
+                a0 = Fb[i0];
+                a1 = Fb[i0 + 1];
                 amp = fmaf(f, a1 - a0, a0);
               }
               if constexpr (MODE == MODE_FWD) {
-                accb[c] = fmaf(ok ? w : 0.f, amp, accb[c]);
+                accb[c] = fmaf(ok ? wd : 0.f, amp, accb[c]);
                 fcnt[c] += ok ? 1 : 0;
               } else {
-                // d/dv of w * amp with masks held (isotropic, no antialias)
-                const float sqv = __builtin_amdgcn_sqrtf(q[c]);  // 1/v
-                const float hsr = h2[c].x * rs * rs, hrr = h2[c].y * rr * rr;
-                float dw;
-                if (a.weight_mode == PSTM_WEIGHT_FULL) dw = -2.f * sqv * w * (1.f - q[c] * (hsr + hrr));
-                else if (a.weight_mode == PSTM_WEIGHT_SPREADING)
-                  dw = -2.f * sqv * w * (1.f - 0.5f * q[c] * (hsr + hrr));
-                else if (a.weight_mode == PSTM_WEIGHT_OBLIQUITY) dw = sqv * w * q[c] * (hsr + hrr);
-                else dw = 0.f;
+                // d/dv of cw * wd * amp with masks held (isotropic, no antialias)
+                const float Q = qd[c] * fmaf(h2.x, rs * rs, h2.y * (rr * rr));  // q (h_s^2/t_s^2 + h_r^2/t_r^2)
+                float dwf;  // (d/dv of the weight) / weight
+                if (wmode == PSTM_WEIGHT_FULL) dwf = -2.f * sv[c] * (1.f - Q);
+                else if (wmode == PSTM_WEIGHT_SPREADING) dwf = -2.f * sv[c] * (1.f - 0.5f * Q);
+                else if (wmode == PSTM_WEIGHT_OBLIQUITY) dwf = sv[c] * Q;
+                else dwf = 0.f;
                 const bool clamped = !(u_raw >= 0.f && u_raw <= a.umax);
-                const float dt_dv = -sqv * q[c] * fmaf(h2[c].x, rs, h2[c].y * rr);
+                const float dudv = -sv[c] * qd[c] * fmaf(h2.x, rs, h2.y * rr);  // d(t/dt)/dv
                 const float slope = clamped ? 0.f : (a1 - a0);
-                const float d = fmaf(w * slope, dt_dv * a.inv_dt, dw * amp);
+                const float d = wd * fmaf(slope, dudv, dwf * amp);
                 accb[c] += ok ? d : 0.f;
               }
             } else {
               // adjoint: scatter w * m into the window (transpose of the taps)
-              const float gsc = ok ? w * m_in[c] : 0.f;
+              const float gsc = ok ? wd * m_in[c] : 0.f;
               if (gsc != 0.f) {
                 if constexpr (AA) {
                   const float um = u - L, up = u + L;
                   const int im = (int)floorf(um), ip = (int)up;
                   const float fm = um - (float)im, fp = up - (float)ip;
                   const double gc = 2.0 * (double)(gsc * iL2), ge = -(double)(gsc * iL2);
-                  atomicAdd(&Wn[base + i0], (WT)(gc * (1.0 - f)));
-                  atomicAdd(&Wn[base + i0 + 1], (WT)(gc * f));
-                  atomicAdd(&Wn[base + im], (WT)(ge * (1.0 - fm)));
-                  atomicAdd(&Wn[base + im + 1], (WT)(ge * fm));
-                  atomicAdd(&Wn[base + ip], (WT)(ge * (1.0 - fp)));
-                  atomicAdd(&Wn[base + ip + 1], (WT)(ge * fp));
+                  atomicAdd(&Ab[i0], gc * (1.0 - f));
+                  atomicAdd(&Ab[i0 + 1], gc * f);
+                  atomicAdd(&Ab[im], ge * (1.0 - fm));
+                  atomicAdd(&Ab[im + 1], ge * fm);
+                  atomicAdd(&Ab[ip], ge * (1.0 - fp));
+                  atomicAdd(&Ab[ip + 1], ge * fp);
                 } else {
-                  atomicAdd(&Wn[base + i0], (WT)(gsc * (1.f - f)));
-                  atomicAdd(&Wn[base + i0 + 1], (WT)(gsc * f));
+                  atomicAdd(&Fb[i0], gsc * (1.f - f));
+                  atomicAdd(&Fb[i0 + 1], gsc * f);
                 }
               }
             }
@@ -443,13 +513,13 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
           if (a.flags & PSTM_FLAG_KAHAN) {
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
-              neumaier(acc[c], cmp[c], accb[c]);
+              neumaier(acc[c], cmp[c], accb[c] * cw[c]);
               accb[c] = 0.f;
             }
           } else {
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
-              acc[c] += accb[c];
+              acc[c] = fmaf(accb[c], cw[c], acc[c]);
               accb[c] = 0.f;
             }
           }
@@ -458,7 +528,8 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
         if constexpr (MODE == MODE_ADJ) {
           if constexpr (AA) {
             // transpose of the double integration: the same symmetric operator
-            for (int b = wave; b < nb; b += kWaves) integrate_window(Wn + S.b_off[b], S.q_len[qs + b], lane);
+            for (int b = wave; b < nb; b += kWaves)
+              integrate_window(Wa + S.b_off[b], nullptr, S.q_len[qs + b], lane);
             __syncthreads();
           }
           // flush the windows into the output traces
@@ -469,7 +540,7 @@ __global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
             float* row = a.traces_out + (int64_t)id * a.n_samples;
             for (int s = lane; s < len; s += 64) {
               const int dst = w0 + s;
-              const float v = (float)Wn[off + s];
+              const float v = AA ? (float)Wa[off + s] : Wf[off + s];
               if (dst >= 0 && dst < a.n_samples && v != 0.f) atomicAdd(&row[dst], v);
             }
           }

@@ -148,16 +148,19 @@ def test_empty_and_unreachable_traces():
     assert pl.kept() == 0 and float(fold.sum()) == 0
 
 
-def test_long_traces_near_lds_limit():
-    s = S.tiny_survey(seed=15, n_bins=6, n_classes=1, n_samples=5800, n_img=4, aperture=300.0)
-    p = s.params.replace(n_tau=700)
+@pytest.mark.parametrize("n_samples,flags", [(9000, 0), (5000, FLAG_AA)])
+def test_long_traces_near_lds_limit(n_samples, flags):
+    # windows of whole traces close to the LDS sample capacity (9082 plain,
+    # 5099 with antialias at aa_max 16)
+    s = S.tiny_survey(seed=15, n_bins=6, n_classes=1, n_samples=n_samples, n_img=4, aperture=300.0)
+    p = s.params.replace(n_tau=700, flags=flags)
     vel = s.vel[:700].contiguous()
     from seisproc_amd.processors.migration.hip_ops import PstmPlan
     plan = PstmPlan(p, s.src_xy, s.rcv_xy)
     img, fold = plan.forward(s.traces.to(DEV), vel.to(DEV))
     sx, sy, rx, ry = s.numpy_geometry()
     ref, rfold = oracle.forward(p, s.traces.double().numpy(), sx, sy, rx, ry, vel.double().numpy())
-    _compare("long-traces", img, fold, ref, rfold)
+    _compare(f"long-traces-{n_samples}", img, fold, ref, rfold)
 
 
 def test_input_validation_errors():
