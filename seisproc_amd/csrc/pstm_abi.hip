@@ -73,7 +73,7 @@ static int validate(const pstm_params_t* p) {
   // A staged window never exceeds n_samples + 2 (pad + margin) + 1 samples.
   const bool aa = (p->flags & PSTM_FLAG_AA) != 0;
   const int64_t cap = min_cap_elems(aa);
-  if ((int64_t)p->n_samples + 2 * (pad + kWindowMargin) + 2 > cap)
+  if (whole_window(p, pad) > cap)
     return set_error(PSTM_ERR_UNSUPPORTED, "n_samples %d too long for the LDS sample buffer (%lld samples%s)",
                      p->n_samples, (long long)cap, aa ? " with antialias" : "");
   if ((int64_t)p->n_il * p->n_xl * (int64_t)p->n_tau > ((int64_t)1 << 40))
@@ -129,17 +129,13 @@ int compute_layout(const pstm_params_t* p, PlanLayout* L) {
 }
 
 // Tile shape of a call: 2D lines use 8 columns along the line; 3D images use
-// 4 x 2 columns (8 per lane: ~150 VGPRs, 3 waves per SIMD on the fast path,
-// measured faster than 4 x 4 at 237 VGPRs).  PSTM_TILE=4x4 selects 4 x 4 for
-// calls without antialias or VTI.
+// 4 x 2 columns (8 per lane: ~130 VGPRs, 3 waves per SIMD on the fast path;
+// 4 x 4 tiles need ~240 VGPRs and measured slower).
 static void choose_shape(const pstm_params_t* p, int* tx, int* ty) {
   if (p->n_xl == 1) { *tx = 8; *ty = 1; return; }
   if (p->n_il == 1) { *tx = 1; *ty = 8; return; }
-  const bool heavy = (p->flags & (PSTM_FLAG_AA | PSTM_FLAG_VTI)) != 0;
   *tx = 4;
   *ty = 2;
-  const char* env = getenv("PSTM_TILE");
-  if (env && strcmp(env, "4x4") == 0 && !heavy) *ty = 4;
 }
 
 void fill_tile_args(const pstm_params_t* p, const PlanLayout& L, const void* ws, TileArgs* a) {

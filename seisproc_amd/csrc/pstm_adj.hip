@@ -22,9 +22,7 @@ int launch_adjoint(const pstm_params_t* p, const PlanLayout& L, const TileArgs& 
              : PSTM_ADJ(TX, TY, true, false, GENV))     \
       : (vti ? PSTM_ADJ(TX, TY, false, true, GENV)      \
              : PSTM_ADJ(TX, TY, false, false, GENV)))
-  if (a.tile_x == 4 && a.tile_y == 4 && !aa && !vti)
-    rc = gen ? PSTM_ADJ(4, 4, false, false, true) : PSTM_ADJ(4, 4, false, false, false);
-  else if (a.tile_x == 4 && a.tile_y == 2) rc = gen ? PSTM_ADJ4(4, 2, true) : PSTM_ADJ4(4, 2, false);
+  if (a.tile_x == 4 && a.tile_y == 2) rc = gen ? PSTM_ADJ4(4, 2, true) : PSTM_ADJ4(4, 2, false);
   else if (a.tile_x == 8 && a.tile_y == 1) rc = PSTM_ADJ4(8, 1, true);
   else if (a.tile_x == 1 && a.tile_y == 8) rc = PSTM_ADJ4(1, 8, true);
   else rc = set_error(PSTM_ERR_INVALID, "unsupported tile shape %dx%d", a.tile_x, a.tile_y);

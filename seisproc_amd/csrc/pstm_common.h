@@ -16,17 +16,19 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kTileTau = kThreads;
 
+// Margin (samples) added on each side of a staged window beyond the
+// rigorous traveltime bounds (absorbs FP32 rounding of the bounds).
+constexpr int kWindowMargin = 2;
+
 constexpr int kMaxBatch = 64;     // traces per sub-batch
 constexpr int kQueueCap = 320;    // candidate queue entries (>= kMaxBatch - 1 + kThreads)
 
-// LDS budget of one tile workgroup (160 KB per CU, allocated in 512-byte
-// granules): the plain kernels run 3 workgroups per CU, the antialias
-// kernels (more registers) 2.  Three workgroups of 54272 bytes (159 KB)
-// were measured to fit only two per CU; 53760 bytes (157.5 KB for three)
-// fit three.
-constexpr int kLdsBudget = 53760;
-constexpr int kLdsBudgetAA = 80896;
-static_assert(kLdsBudget % 512 == 0 && kLdsBudgetAA % 512 == 0, "512-byte LDS granules");
+// LDS budget of one tile workgroup for OCC (2 or 3) workgroups per CU
+// (160 KB per CU, allocated in 512-byte granules).  Three workgroups of
+// 54272 bytes (159 KB) were measured to fit only two per CU; 53760 bytes
+// (157.5 KB for three) fit three.
+constexpr int lds_budget(int occ) { return occ >= 3 ? 53760 : 80896; }
+static_assert(lds_budget(3) % 512 == 0 && lds_budget(2) % 512 == 0, "512-byte LDS granules");
 
 // Tile-kernel LDS besides the staged sample windows.
 struct TileShared {
@@ -56,26 +58,27 @@ struct PairShared {
 // Staged-window capacity in floats and in window elements.  A window
 // element is one FP32 sample, or with the antialias filter the FP64 double
 // integral plus the FP32 single integral (three floats).
-template <int NC, bool AA>
+template <int NC, bool AA, int OCC>
 constexpr int sample_cap_floats() {
-  return (((AA ? kLdsBudgetAA : kLdsBudget) - (int)sizeof(TileShared) - (int)sizeof(PairShared<NC, AA>)) / 4) &
-         ~63;
+  return ((lds_budget(OCC) - (int)sizeof(TileShared) - (int)sizeof(PairShared<NC, AA>)) / 4) & ~63;
 }
-template <int NC, bool AA>
+template <int NC, bool AA, int OCC>
 constexpr int sample_cap_elems() {
-  return AA ? sample_cap_floats<NC, AA>() / 3 : sample_cap_floats<NC, AA>();
+  return AA ? sample_cap_floats<NC, AA, OCC>() / 3 : sample_cap_floats<NC, AA, OCC>();
 }
-// Smallest capacity over the launched tile shapes (8 and 16 columns;
-// antialias only with 8).
+// Workgroups per CU of each kernel form: 3 for the plain tiles, 2 for the
+// antialias kernels that take any trace length (the antialias fast path
+// also has a 3-per-CU form for short traces) and for the velocity gradient.
+constexpr int default_occ(bool aa) { return aa ? 2 : 3; }
+// Capacity every call can rely on (8-column tiles).
 constexpr int min_cap_elems(bool aa) {
-  return aa ? sample_cap_elems<8, true>()
-            : (sample_cap_elems<8, false>() < sample_cap_elems<16, false>() ? sample_cap_elems<8, false>()
-                                                                             : sample_cap_elems<16, false>());
+  return aa ? sample_cap_elems<8, true, default_occ(true)>() : sample_cap_elems<8, false, default_occ(false)>();
+}
+// Window length (elements) of a whole trace with its padding.
+inline int64_t whole_window(const pstm_params_t* p, int aa_pad) {
+  return (int64_t)p->n_samples + 2 * (aa_pad + kWindowMargin) + 2;
 }
 
-// Margin (samples) added on each side of a staged window beyond the
-// rigorous traveltime bounds (absorbs FP32 rounding of the bounds).
-constexpr int kWindowMargin = 2;
 
 // Plan layout: everything the kernels need about the trace binning, derived
 // deterministically from the job parameters on the host.

@@ -6,13 +6,12 @@ namespace pstm {
 
 int launch_vel_grad(const pstm_params_t* p, const PlanLayout& L, const TileArgs& a, hipStream_t s) {
   int rc;
-#define PSTM_GRAD(TX, TY, DTV) launch_one<MODE_GRAD, TX, TY, DTV, false, false, true>(a, s)
+#define PSTM_GRAD(TX, TY, DTV) launch_one<MODE_GRAD, TX, TY, DTV, false, false, true, 2>(a, s)
 #define PSTM_GRAD3(TX, TY)                                                      \
   (p->dtype == PSTM_F32 ? PSTM_GRAD(TX, TY, PSTM_F32)                           \
                         : p->dtype == PSTM_BF16 ? PSTM_GRAD(TX, TY, PSTM_BF16)   \
                                                 : PSTM_GRAD(TX, TY, PSTM_F16))
-  if (a.tile_x == 4 && a.tile_y == 4) rc = PSTM_GRAD3(4, 4);
-  else if (a.tile_x == 4 && a.tile_y == 2) rc = PSTM_GRAD3(4, 2);
+  if (a.tile_x == 4 && a.tile_y == 2) rc = PSTM_GRAD3(4, 2);
   else if (a.tile_x == 8 && a.tile_y == 1) rc = PSTM_GRAD3(8, 1);
   else if (a.tile_x == 1 && a.tile_y == 8) rc = PSTM_GRAD3(1, 8);
   else rc = set_error(PSTM_ERR_INVALID, "unsupported tile shape %dx%d", a.tile_x, a.tile_y);

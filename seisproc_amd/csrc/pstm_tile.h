@@ -180,12 +180,14 @@ __device__ __forceinline__ float taper_of(float hm2, float hl2, float tw) {
 //   SPREADING  q (t_s t_r)^-1         = cw prs,    cw = q / dt^2
 //   OBLIQUITY  tau^2/4 (t_s t_r)^-1   = cw prs,    cw = tau^2/4 / dt^2
 //   NONE       1                      = cw,        cw = 1
-template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN>
-__global__ __launch_bounds__(kThreads) void k_tile(TileArgs a) {
+//
+// OCC: workgroups per CU the kernel is built for (registers and LDS).
+template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN, int OCC>
+__global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   constexpr int NC = TX * TY;
-  static_assert(NC == 4 || NC == 8 || NC == 16, "columns per tile: 4, 8 or 16");
-  constexpr int CAPF = sample_cap_floats<NC, AA>();  // staged-window floats
-  constexpr int CAPE = sample_cap_elems<NC, AA>();   // staged-window elements
+  static_assert(NC == 8, "columns per tile: 8");
+  constexpr int CAPF = sample_cap_floats<NC, AA, OCC>();  // staged-window floats
+  constexpr int CAPE = sample_cap_elems<NC, AA, OCC>();   // staged-window elements
   __shared__ TileShared S;
   __shared__ PairShared<NC, AA> P;
   __shared__ __align__(16) float samp[CAPF];
@@ -669,12 +671,13 @@ inline int64_t tile_blocks(const TileArgs& a) {
   return (int64_t)8 * chunk * a.n_tblocks;
 }
 
-template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN>
+template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN, int OCC = default_occ(AA)>
 int launch_one(const TileArgs& a, hipStream_t s) {
   const int64_t blocks = tile_blocks(a);
   if (blocks <= 0) return PSTM_OK;
   if (blocks > 0x7fffffff) return set_error(PSTM_ERR_INVALID, "too many tiles (%lld)", (long long)blocks);
-  hipLaunchKernelGGL((k_tile<MODE, TX, TY, DT, AA, VTI, GEN>), dim3((unsigned)blocks), dim3(kThreads), 0, s, a);
+  hipLaunchKernelGGL((k_tile<MODE, TX, TY, DT, AA, VTI, GEN, OCC>), dim3((unsigned)blocks), dim3(kThreads), 0, s,
+                     a);
   return PSTM_OK;
 }
 
@@ -703,8 +706,12 @@ int launch_forward_impl(const pstm_params_t* p, const PlanLayout& L, const TileA
              : PSTM_FWD(TX, TY, true, false, GENV))     \
       : (vti ? PSTM_FWD(TX, TY, false, true, GENV)      \
              : PSTM_FWD(TX, TY, false, false, GENV)))
-  if (a.tile_x == 4 && a.tile_y == 4 && !aa && !vti)
-    return gen ? PSTM_FWD(4, 4, false, false, true) : PSTM_FWD(4, 4, false, false, false);
+  if (a.tile_x == 4 && a.tile_y == 2 && aa && !gen &&
+      whole_window(p, a.aa_pad) <= sample_cap_elems<8, true, 3>()) {
+    // short traces: antialias fast path at 3 workgroups per CU
+    return vti ? launch_one<MODE_FWD, 4, 2, DT, true, true, false, 3>(a, s)
+               : launch_one<MODE_FWD, 4, 2, DT, true, false, false, 3>(a, s);
+  }
   if (a.tile_x == 4 && a.tile_y == 2) return gen ? PSTM_FWD4(4, 2, true) : PSTM_FWD4(4, 2, false);
   if (a.tile_x == 8 && a.tile_y == 1) return PSTM_FWD4(8, 1, true);
   if (a.tile_x == 1 && a.tile_y == 8) return PSTM_FWD4(1, 8, true);
