@@ -22,7 +22,19 @@ int launch_adjoint(const pstm_params_t* p, const PlanLayout& L, const TileArgs& 
              : PSTM_ADJ(TX, TY, true, false, GENV))     \
       : (vti ? PSTM_ADJ(TX, TY, false, true, GENV)      \
              : PSTM_ADJ(TX, TY, false, false, GENV)))
-  if (a.tile_x == 4 && a.tile_y == 2) rc = gen ? PSTM_ADJ4(4, 2, true) : PSTM_ADJ4(4, 2, false);
+  if (a.tile_x == 4 && a.tile_y == 2 && !aa) {
+    // 3D without antialias: 8 x 4-column tiles, so that each staged trace
+    // window (flushed to the traces with global atomics) serves 32 columns
+    TileArgs b = a;
+    b.tile_x = 8;
+    b.tile_y = 4;
+    b.n_tiles_x = (p->n_il + 7) / 8;
+    b.n_tiles_y = (p->n_xl + 3) / 4;
+    rc = gen ? (vti ? launch_one<MODE_ADJ, 8, 4, PSTM_F32, false, true, true, 2>(b, s)
+                    : launch_one<MODE_ADJ, 8, 4, PSTM_F32, false, false, true, 2>(b, s))
+             : (vti ? launch_one<MODE_ADJ, 8, 4, PSTM_F32, false, true, false, 2>(b, s)
+                    : launch_one<MODE_ADJ, 8, 4, PSTM_F32, false, false, false, 2>(b, s));
+  } else if (a.tile_x == 4 && a.tile_y == 2) rc = gen ? PSTM_ADJ4(4, 2, true) : PSTM_ADJ4(4, 2, false);
   else if (a.tile_x == 8 && a.tile_y == 1) rc = PSTM_ADJ4(8, 1, true);
   else if (a.tile_x == 1 && a.tile_y == 8) rc = PSTM_ADJ4(1, 8, true);
   else rc = set_error(PSTM_ERR_INVALID, "unsupported tile shape %dx%d", a.tile_x, a.tile_y);

@@ -100,15 +100,16 @@ __device__ __forceinline__ double lane63(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// OR over each aligned group of NC (4, 8 or 16) lanes; the result is valid in
-// the group's last lane.
+// OR over each aligned group of NC (4, 8, 16 or 32) lanes; the result is
+// valid in the group's last lane.
 template <int NC>
 __device__ __forceinline__ int group_or(int v) {
-  static_assert(NC == 4 || NC == 8 || NC == 16, "group of 4, 8 or 16 lanes");
+  static_assert(NC == 4 || NC == 8 || NC == 16 || NC == 32, "group of 4, 8, 16 or 32 lanes");
   v |= dpp_move<kDppQuadXor1>(v);
   v |= dpp_move<kDppQuadXor2>(v);
   if constexpr (NC >= 8) v |= dpp_move<kDppRowShr4>(v);
   if constexpr (NC >= 16) v |= dpp_move<kDppRowShr8>(v);
+  if constexpr (NC >= 32) v |= dpp_move<kDppRowBcast15, 0xa>(v);
   return v;
 }
 
@@ -185,7 +186,7 @@ __device__ __forceinline__ float taper_of(float hm2, float hl2, float tw) {
 template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN, int OCC>
 __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   constexpr int NC = TX * TY;
-  static_assert(NC == 8, "columns per tile: 8");
+  static_assert(NC == 8 || (NC == 32 && MODE == MODE_ADJ), "columns per tile: 8 (32 for the adjoint)");
   constexpr int CAPF = sample_cap_floats<NC, AA, OCC>();  // staged-window floats
   constexpr int CAPE = sample_cap_elems<NC, AA, OCC>();   // staged-window elements
   __shared__ TileShared S;

@@ -363,7 +363,7 @@ class BSplineVelocity:
 
     def fit(self, v: torch.Tensor) -> torch.Tensor:
         """Least-squares coefficients reproducing a grid velocity."""
-        px, py, pt = (torch.linalg.pinv(B.double()).float() for B in (self.Bx, self.By, self.Bt))
+        px, py, pt = (torch.linalg.pinv(B.double().cpu()).float().to(B.device) for B in (self.Bx, self.By, self.Bt))
         a = torch.matmul(v, pt.t())
         a = torch.matmul(py, a)
         a = torch.matmul(px, a.reshape(v.shape[0], -1))

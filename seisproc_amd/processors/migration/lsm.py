@@ -5,8 +5,10 @@ migration L (forward kernel: data -> image).  Solves
     min_m || D m - d ||^2 (+ damp ||m||^2)
 with conjugate gradients on L D m = L d; every iteration costs one adjoint
 and one forward launch, all on the GPU with FP64 reductions of the inner
-products.  The reference has no least-squares migration; this is the
-differentiation form of the north star (30 iterations of forward + adjoint).
+products.  The two operators are the torch.autograd.Function objects of
+autograd.py (applied without recording a graph), so the CG iterations use
+exactly the pair whose gradients autograd returns.  The reference has no
+least-squares migration.
 """
 from __future__ import annotations
 
@@ -17,6 +19,7 @@ from typing import Callable, List, Optional
 import torch
 
 from ...abi_types import FLAG_KAHAN
+from .autograd import demigrate, migrate
 
 
 @dataclass
@@ -40,11 +43,12 @@ def cgls(plan, data: torch.Tensor, vel: torch.Tensor, n_iter: int = 30, *, eta: 
     fl = (plan.params.flags if flags is None else flags) & ~FLAG_KAHAN
 
     def L(d):  # data -> image
-        img, _ = plan.forward(d, vel, eta=eta, with_fold=False, flags=fl)
-        return img
+        with torch.no_grad():
+            return migrate(d, vel, plan, flags=fl, eta=eta)
 
     def Lt(m):  # image -> data
-        return plan.adjoint(m, vel, eta=eta, flags=fl)
+        with torch.no_grad():
+            return demigrate(m, vel, plan, flags=fl, eta=eta)
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()

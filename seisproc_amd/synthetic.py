@@ -348,6 +348,54 @@ def config5_vti(seed: int = 5, n_bins: int = 200, n_classes: int = 60, class_min
                            class_sizes=sizes, info=dict(diffractors=diff))
 
 
+def reflectivity(gen: torch.Generator, n_il: int, n_xl: int, n_tau: int, dt: float, p_spike: float = 0.01,
+                 f_peak: float = 25.0, device="cpu") -> torch.Tensor:
+    """Sparse Bernoulli(p_spike) spikes with N(0, 1) amplitudes on an
+    [il][xl][tau] grid, convolved along time with a zero-phase Ricker."""
+    spikes = torch.rand((n_il, n_xl, n_tau), generator=gen) < p_spike
+    amp = (torch.randn((n_il, n_xl, n_tau), generator=gen) * spikes).to(device)
+    half = int(math.ceil(1.5 / (f_peak * dt)))
+    w = ricker(dt * torch.arange(-half, half + 1, dtype=torch.float64), f_peak).tolist()
+    r = torch.zeros_like(amp)
+    for k, wk in enumerate(w):  # r[t] = sum_k w[k] amp[t - half + k] (w is symmetric)
+        s = half - k
+        if s >= 0:
+            r[..., s:] += wk * amp[..., :n_tau - s]
+        else:
+            r[..., :n_tau + s] += wk * amp[..., -s:]
+    return r
+
+
+def config4_lsm(seed: int = 4, n_img: int = 256, n_tau: int = 751, d_img: float = 20.0, n_classes: int = 16,
+                aperture: float = 2000.0, snr: float = 10.0, p_spike: float = 0.01,
+                device="cuda") -> SyntheticSurvey:
+    """Differentiation form: reflectivity n_img^2 x n_tau at d_img m and 2 ms
+    (Bernoulli(p_spike) N(0,1) spikes convolved with a 25 Hz Ricker);
+    n_classes offset classes 0:100:(100 (n_classes - 1)) m, one trace per image
+    bin per class (random azimuth), n_tau samples at 2 ms; v_rms rising
+    linearly from 1500 to 3500 m/s over the record; data = demigration
+    (adjoint kernel) of the reflectivity plus Gaussian noise at an RMS
+    signal-to-noise ratio of snr; aperture 2000 m, 60 deg.  Needs the GPU
+    (the data are modelled by the adjoint kernel)."""
+    from .processors.migration.hip_ops import PstmPlan
+    gen = torch.Generator().manual_seed(seed)
+    dt = 0.002
+    offsets = [100.0 * i for i in range(n_classes)]
+    src, rcv, sizes = _cmp_grid_geometry(gen, n_img, n_img, d_img, offsets)
+    p = _image_params(n_img, n_img, n_tau, d_img, d_img, n_tau, dt, aperture, 60.0, 0, n_traces=src.shape[0])
+    tmax = dt * (n_tau - 1)
+    vf = VrmsField(1500.0, 2000.0 / tmax)
+    vel = vf.on_grid(p, device)
+    m_true = reflectivity(gen, n_img, n_img, n_tau, dt, p_spike, 25.0, device)
+    plan = PstmPlan(p, src.to(device), rcv.to(device), device=torch.device(device))
+    data = plan.adjoint(m_true, vel, flags=0)
+    rms = float(data.double().pow(2).mean().sqrt())
+    ng = torch.Generator(device=device).manual_seed(seed + 1)
+    data += (rms / snr) * torch.randn(data.shape, generator=ng, device=device, dtype=torch.float32)
+    return SyntheticSurvey("cfg4-lsm", p, src.to(device), rcv.to(device), data, vel, class_sizes=sizes,
+                           info=dict(reflectivity=m_true, plan=plan, snr=snr))
+
+
 def tiny_survey(seed: int = 0, n_bins: int = 12, n_classes: int = 3, n_samples: int = 301,
                 n_img: int = 10, d_img: float = 25.0, aperture: float = 400.0, flags: int = 0,
                 lateral: bool = False, device="cpu", utm: bool = False) -> SyntheticSurvey:
