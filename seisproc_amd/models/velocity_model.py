@@ -170,9 +170,14 @@ class VelocityModel:
         tt, vr = depth_interval_to_vrms_table(self.z_axis, self.data.astype(np.float64))
         return np.interp(t_s, tt, vr)
 
-    def vrms_on_grid(self, p, device="cuda", honor_lateral: bool = True) -> torch.Tensor:
+    def vrms_on_grid(self, p, device="cuda", honor_lateral: bool = True, inline_start: int = 1,
+                     xline_start: int = 1) -> torch.Tensor:
         """v_rms at every output sample of the image described by the
-        PstmParams `p` (world coordinates from its origin and step vectors)."""
+        PstmParams `p`.  Lateral axes are world coordinates (from the grid's
+        origin and step vectors), or, for tables read by lateral number
+        (``metadata['lateral_axes']``), the image columns' inline/xline
+        numbers (inline_start + il, xline_start + xl) or CDP numbers along a
+        line (inline_start + il)."""
         tau = p.tau0 + p.dtau * np.arange(p.n_tau)
         if self._type in (VelocityType.CONSTANT, VelocityType.V_OF_Z):
             return torch.as_tensor(self.rms_velocity_at_time(tau), dtype=torch.float32, device=device)
@@ -180,8 +185,13 @@ class VelocityModel:
             raise NotImplementedError("laterally varying depth models must be converted to v_rms(t) first")
         il = np.arange(p.n_il, dtype=np.float64)
         xl = np.arange(p.n_xl, dtype=np.float64)
-        X = p.origin_x + il[:, None] * p.il_dx + xl[None, :] * p.xl_dx
-        Y = p.origin_y + il[:, None] * p.il_dy + xl[None, :] * p.xl_dy
+        lateral = self.metadata.get("lateral_axes")
+        if lateral in ("inline_xline", "cdp"):
+            X = (inline_start + il[:, None]) + 0.0 * xl[None, :]
+            Y = (xline_start + xl[None, :]) + 0.0 * il[:, None]
+        else:
+            X = p.origin_x + il[:, None] * p.il_dx + xl[None, :] * p.xl_dx
+            Y = p.origin_y + il[:, None] * p.il_dy + xl[None, :] * p.xl_dy
         out = np.empty((p.n_il, p.n_xl, p.n_tau), dtype=np.float32)
         for i in range(p.n_il):
             zz = np.broadcast_to(tau[None, :], (p.n_xl, p.n_tau))
