@@ -84,7 +84,8 @@ def _workload(args, torch):
                "velocity": "v_rms = (1500 + 600 t)(1 +- 5% lateral)"}
         forms = ["fp32", "bf16"]
     else:
-        s = S.config5_vti(device=dev)
+        # --classes n: n classes over the same 0-4000 m offset range (a launch must end within minutes)
+        s = S.config5_vti(device=dev, n_classes=args.classes or 60)
         cfg = {"workload": "pstm3d-vti-largest", "traces": s.n_traces, "classes": len(s.class_sizes),
                "samples": s.params.n_samples, "image": [s.params.n_il, s.params.n_xl, s.params.n_tau],
                "bin_m": 12.5, "aperture_m": 4000.0, "antialias": True, "vti": True}
@@ -201,9 +202,15 @@ def _bench_lsm(args, torch):
           file=sys.stderr, flush=True)
     if args.warmup:
         cgls(plan, data, vel, n_iter=args.warmup, flags=0)
-    stamps = []
-    res = cgls(plan, data, vel, n_iter=args.steps, flags=0,
-               callback=lambda it, rn, gn: stamps.append(time.perf_counter()))
+    stamps = [time.perf_counter()]
+
+    def on_iteration(it, rn, gn):
+        stamps.append(time.perf_counter())
+        print(f"[bench] CG iteration {it}: residual {rn:.4e} ({stamps[-1] - stamps[-2]:.1f} s)", file=sys.stderr,
+              flush=True)
+
+    res = cgls(plan, data, vel, n_iter=args.steps, flags=0, callback=on_iteration)
+    stamps = stamps[1:]
     per_it = [b - a for a, b in zip(stamps, stamps[1:])]
     sec_it = sum(per_it) / len(per_it) if per_it else res.seconds_per_iteration
     rate = 2.0 * contrib / sec_it / 1e9
@@ -244,7 +251,7 @@ def _bench_lsm(args, torch):
         from oracle.ref_torch import demigrate_kirchhoff_full_cpu, migrate_kirchhoff_full_cpu
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         torch.set_num_threads(threads)
-        stride = max(args.cpu_tau_stride, 1) * 2
+        stride = max(args.cpu_tau_stride, 1) * 4
         sel, geo, ox, oy, tau_s = _cpu_sample_grid(s, stride, torch)
         v_top = 1500.0  # the restated reference loop takes one velocity: its value is irrelevant to the rate
         depth = v_top * tau_s / 2.0
@@ -346,18 +353,24 @@ def main():
         w = args.warmup if form == main_form else 0
         k = args.steps if form == main_form else 1
         tr, sc = data[form]
-        for _ in range(w):
+        for i in range(w):
+            t_w = time.perf_counter()
             step(form)
+            torch.cuda.synchronize()
+            print(f"[bench] {form} warm-up {i + 1}/{w}: {time.perf_counter() - t_w:.2f} s", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(k)]
         t0 = time.perf_counter()
-        for e0, e1 in ev:
+        for i, (e0, e1) in enumerate(ev):
             image.zero_()
             comp.zero_()
             fold.zero_()
             e0.record(stream)
             plan.forward(tr, vel, scale=sc, eta=eta, image=image, comp=comp, fold=fold)
             e1.record(stream)
+            e1.synchronize()  # one line per step (long launches)
+            print(f"[bench] {form} step {i + 1}/{k}: kernel {e0.elapsed_time(e1):.1f} ms", file=sys.stderr,
+                  flush=True)
         torch.cuda.synchronize()
         wall = (time.perf_counter() - t0) / k
         kts = sorted(e0.elapsed_time(e1) / 1e3 for e0, e1 in ev)

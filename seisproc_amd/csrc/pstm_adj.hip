@@ -15,14 +15,21 @@ int launch_forward(const pstm_params_t* p, const PlanLayout& L, const TileArgs& 
 int launch_adjoint(const pstm_params_t* p, const PlanLayout& L, const TileArgs& a, hipStream_t s) {
   const bool aa = (p->flags & PSTM_FLAG_AA) != 0, vti = (p->flags & PSTM_FLAG_VTI) != 0;
   const bool gen = !fast_path(p);
+  const int64_t win = whole_window(p, a.aa_pad);
   int rc;
-#define PSTM_ADJ(TX, TY, AAV, VTIV, GENV) launch_one<MODE_ADJ, TX, TY, PSTM_F32, AAV, VTIV, GENV>(a, s)
-#define PSTM_ADJ4(TX, TY, GENV)                         \
-  (aa ? (vti ? PSTM_ADJ(TX, TY, true, true, GENV)       \
-             : PSTM_ADJ(TX, TY, true, false, GENV))     \
-      : (vti ? PSTM_ADJ(TX, TY, false, true, GENV)      \
-             : PSTM_ADJ(TX, TY, false, false, GENV)))
-  if (a.tile_x == 4 && a.tile_y == 2 && !aa) {
+  // Windows accumulate in 64-bit fixed point (integer LDS atomics) whenever
+  // a whole trace window fits at 8 bytes per sample -- always with antialias,
+  // whose FP64 windows have that size; longer plain windows fall back to
+  // FP32 windows and float LDS atomics.
+#define PSTM_ADJ(TX, TY, AAV, VTIV, GENV, OCCV, FXV) \
+  launch_one<MODE_ADJ, TX, TY, PSTM_F32, AAV, VTIV, GENV, OCCV, FXV>(a, s)
+#define PSTM_ADJ4(TX, TY, GENV, OCCV, FXV)                            \
+  (aa ? (vti ? PSTM_ADJ(TX, TY, true, true, GENV, 2, true)            \
+             : PSTM_ADJ(TX, TY, true, false, GENV, 2, true))          \
+      : (vti ? PSTM_ADJ(TX, TY, false, true, GENV, OCCV, FXV)         \
+             : PSTM_ADJ(TX, TY, false, false, GENV, OCCV, FXV)))
+  const bool fx8 = win <= window_cap_elems<8, false, 3, true>();
+  if (a.tile_x == 4 && a.tile_y == 2 && !aa && win <= window_cap_elems<32, false, 2, true>()) {
     // 3D without antialias: 8 x 4-column tiles, so that each staged trace
     // window (flushed to the traces with global atomics) serves 32 columns
     TileArgs b = a;
@@ -30,14 +37,20 @@ int launch_adjoint(const pstm_params_t* p, const PlanLayout& L, const TileArgs& 
     b.tile_y = 4;
     b.n_tiles_x = (p->n_il + 7) / 8;
     b.n_tiles_y = (p->n_xl + 3) / 4;
-    rc = gen ? (vti ? launch_one<MODE_ADJ, 8, 4, PSTM_F32, false, true, true, 2>(b, s)
-                    : launch_one<MODE_ADJ, 8, 4, PSTM_F32, false, false, true, 2>(b, s))
-             : (vti ? launch_one<MODE_ADJ, 8, 4, PSTM_F32, false, true, false, 2>(b, s)
-                    : launch_one<MODE_ADJ, 8, 4, PSTM_F32, false, false, false, 2>(b, s));
-  } else if (a.tile_x == 4 && a.tile_y == 2) rc = gen ? PSTM_ADJ4(4, 2, true) : PSTM_ADJ4(4, 2, false);
-  else if (a.tile_x == 8 && a.tile_y == 1) rc = PSTM_ADJ4(8, 1, true);
-  else if (a.tile_x == 1 && a.tile_y == 8) rc = PSTM_ADJ4(1, 8, true);
-  else rc = set_error(PSTM_ERR_INVALID, "unsupported tile shape %dx%d", a.tile_x, a.tile_y);
+    rc = gen ? (vti ? launch_one<MODE_ADJ, 8, 4, PSTM_F32, false, true, true, 2, true>(b, s)
+                    : launch_one<MODE_ADJ, 8, 4, PSTM_F32, false, false, true, 2, true>(b, s))
+             : (vti ? launch_one<MODE_ADJ, 8, 4, PSTM_F32, false, true, false, 2, true>(b, s)
+                    : launch_one<MODE_ADJ, 8, 4, PSTM_F32, false, false, false, 2, true>(b, s));
+  } else if (a.tile_x == 4 && a.tile_y == 2) {
+    rc = fx8 ? (gen ? PSTM_ADJ4(4, 2, true, 3, true) : PSTM_ADJ4(4, 2, false, 3, true))
+             : (gen ? PSTM_ADJ4(4, 2, true, 3, false) : PSTM_ADJ4(4, 2, false, 3, false));
+  } else if (a.tile_x == 8 && a.tile_y == 1) {
+    rc = fx8 ? PSTM_ADJ4(8, 1, true, 3, true) : PSTM_ADJ4(8, 1, true, 3, false);
+  } else if (a.tile_x == 1 && a.tile_y == 8) {
+    rc = fx8 ? PSTM_ADJ4(1, 8, true, 3, true) : PSTM_ADJ4(1, 8, true, 3, false);
+  } else {
+    rc = set_error(PSTM_ERR_INVALID, "unsupported tile shape %dx%d", a.tile_x, a.tile_y);
+  }
 #undef PSTM_ADJ4
 #undef PSTM_ADJ
   return rc != PSTM_OK ? rc : check_launch("adjoint");

@@ -43,6 +43,7 @@ struct TileShared {
   float hwave[kWaves];          // max hlim^2 of each wave (over its columns and times)
   float qmin[kWaves], qmax[kWaves], tqa[kWaves], tqb[kWaves];
   float emax;                   // max eta over the tile (VTI)
+  float gmax[kWaves];           // adjoint: per-wave bound of one contribution
 };
 
 // Per sub-batch (trace, column) quantities of a tile of NC columns.
@@ -65,6 +66,12 @@ constexpr int sample_cap_floats() {
 template <int NC, bool AA, int OCC>
 constexpr int sample_cap_elems() {
   return AA ? sample_cap_floats<NC, AA, OCC>() / 3 : sample_cap_floats<NC, AA, OCC>();
+}
+// Adjoint windows in 64-bit fixed point (FX) take two floats per element
+// without antialias (with antialias the element already holds 8 bytes).
+template <int NC, bool AA, int OCC, bool FX>
+constexpr int window_cap_elems() {
+  return (FX && !AA) ? sample_cap_floats<NC, AA, OCC>() / 2 : sample_cap_elems<NC, AA, OCC>();
 }
 // Workgroups per CU of each kernel form: 3 for the plain tiles, 2 for the
 // antialias kernels that take any trace length (the antialias fast path

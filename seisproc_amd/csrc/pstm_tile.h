@@ -141,6 +141,19 @@ __device__ __forceinline__ void integrate_window(double* A, float* C, int len, i
   }
 }
 
+// d rounded to the nearest integer as a two's-complement 64-bit value
+// (|d| < 2^62): high word floor(d / 2^32), low word the exact remainder.
+__device__ __forceinline__ unsigned long long to_fixed64(double d) {
+  d = rint(d);
+  const double hi = floor(d * 0x1p-32);
+  const double lo = fma(hi, -0x1p32, d);  // exact, in [0, 2^32)
+  const unsigned long long h = (unsigned long long)(unsigned int)(int)hi;
+  return (h << 32) | (unsigned long long)(unsigned int)lo;
+}
+__device__ __forceinline__ double from_fixed64(long long v) {
+  return fma((double)(int)(v >> 32), 0x1p32, (double)(unsigned int)(v & 0xffffffffll));
+}
+
 template <int DT>
 __device__ __forceinline__ float load_sample(const void* tr, int64_t idx) {
   if constexpr (DT == PSTM_F32) {
@@ -183,12 +196,20 @@ __device__ __forceinline__ float taper_of(float hm2, float hl2, float tw) {
 //   NONE       1                      = cw,        cw = 1
 //
 // OCC: workgroups per CU the kernel is built for (registers and LDS).
-template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN, int OCC>
+// FIXED (adjoint): the windows accumulate in 64-bit fixed point with integer
+// LDS atomics (float LDS atomics run at ~1/3 lane per clock on gfx950,
+// 64-bit integer ones at ~9).  Each workgroup scales by 2^s chosen from a
+// bound G of its own contributions so that one contribution is < 2^47 units
+// and a window sample (<= 2^13 contributions of <= 4 taps) stays < 2^62; the
+// accumulation is exact, each contribution is rounded once at 2^-47 G.
+template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN, int OCC, bool FIXED>
 __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   constexpr int NC = TX * TY;
+  constexpr bool FX = MODE == MODE_ADJ && FIXED;
   static_assert(NC == 8 || (NC == 32 && MODE == MODE_ADJ), "columns per tile: 8 (32 for the adjoint)");
-  constexpr int CAPF = sample_cap_floats<NC, AA, OCC>();  // staged-window floats
-  constexpr int CAPE = sample_cap_elems<NC, AA, OCC>();   // staged-window elements
+  static_assert(!FIXED || MODE == MODE_ADJ, "fixed-point windows are an adjoint form");
+  constexpr int CAPF = sample_cap_floats<NC, AA, OCC>();         // staged-window floats
+  constexpr int CAPE = window_cap_elems<NC, AA, OCC, FX>();      // staged-window elements
   __shared__ TileShared S;
   __shared__ PairShared<NC, AA> P;
   __shared__ __align__(16) float samp[CAPF];
@@ -197,6 +218,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   [[maybe_unused]] float* const Wf = samp;
   [[maybe_unused]] double* const Wa = reinterpret_cast<double*>(samp);
   [[maybe_unused]] float* const Wc = samp + 2 * CAPE;
+  [[maybe_unused]] long long* const Wl = reinterpret_cast<long long*>(samp);  // adjoint, fixed point
 
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
@@ -316,6 +338,28 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
                                                  : 0.f;
     }
   }
+  [[maybe_unused]] double fx_scale = 0.0, fx_inv = 0.0;
+  if constexpr (FX) {
+    // bound of one contribution: |m cw| times the largest weight power
+    // (rs, rr <= 1/sqrt(tqs)), times 4 for the antialias taps
+    const float wdmax = wmode == PSTM_WEIGHT_FULL ? 1.0f / (tqs * tqs)
+                                                  : (wmode == PSTM_WEIGHT_NONE ? 1.0f : 1.0f / tqs);
+    float gl = 0.f;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) gl = fmaxf(gl, fabsf(m_in[c]) * wdmax);
+    gl = wave_max(AA ? 4.f * gl : gl);
+    if (lane == 0) S.gmax[wave] = gl;
+    __syncthreads();
+    float G = 0.f;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) G = fmaxf(G, S.gmax[w]);
+    if (!(G > 0.f)) return;  // an all-zero image tile scatters nothing (uniform over the workgroup)
+    G = fminf(G, 3.0e38f);
+    int e;
+    (void)frexpf(G, &e);  // G < 2^e
+    fx_scale = ldexp(1.0, 47 - e);
+    fx_inv = ldexp(1.0, e - 47);
+  }
 
   const bool use_taper = GEN ? (a.flags & PSTM_FLAG_TAPER) != 0 : false;
   const bool zero_out = GEN ? (a.flags & PSTM_FLAG_ZERO_OUTSIDE) != 0 : false;
@@ -368,7 +412,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           const int w0 = S.q_w0[e], len = S.q_len[e], off = S.b_off[b];
           if constexpr (MODE == MODE_ADJ) {
             for (int s = lane; s < len; s += 64) {
-              if constexpr (AA) Wa[off + s] = 0.0;
+              if constexpr (FX) Wl[off + s] = 0;
+              else if constexpr (AA) Wa[off + s] = 0.0;
               else Wf[off + s] = 0.f;
             }
           } else {
@@ -419,6 +464,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           [[maybe_unused]] float* const Fb = Wf + base;
           [[maybe_unused]] double* const Ab = Wa + base;
           [[maybe_unused]] float* const Cb = Wc + base;
+          [[maybe_unused]] unsigned long long* const Lb = reinterpret_cast<unsigned long long*>(Wl + base);
 #pragma unroll
           for (int c = 0; c < NC; ++c) {
             const float2 h2 = P.h2[b][c];
@@ -493,7 +539,24 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               // adjoint: scatter w * m into the window (transpose of the taps)
               const float gsc = ok ? wd * m_in[c] : 0.f;
               if (gsc != 0.f) {
-                if constexpr (AA) {
+                if constexpr (FX) {
+                  const double g = (double)gsc * fx_scale, gf = g * (double)f;
+                  if constexpr (AA) {
+                    const float um = u - L, up = u + L;
+                    const int im = (int)floorf(um), ip = (int)up;
+                    const double fm = (double)(um - (float)im), fp = (double)(up - (float)ip);
+                    const double gc = 2.0 * g * (double)iL2, ge = -g * (double)iL2, gcf = gc * (double)f;
+                    atomicAdd(&Lb[i0], to_fixed64(gc - gcf));
+                    atomicAdd(&Lb[i0 + 1], to_fixed64(gcf));
+                    atomicAdd(&Lb[im], to_fixed64(ge - ge * fm));
+                    atomicAdd(&Lb[im + 1], to_fixed64(ge * fm));
+                    atomicAdd(&Lb[ip], to_fixed64(ge - ge * fp));
+                    atomicAdd(&Lb[ip + 1], to_fixed64(ge * fp));
+                  } else {
+                    atomicAdd(&Lb[i0], to_fixed64(g - gf));
+                    atomicAdd(&Lb[i0 + 1], to_fixed64(gf));
+                  }
+                } else if constexpr (AA) {
                   const float um = u - L, up = u + L;
                   const int im = (int)floorf(um), ip = (int)up;
                   const float fm = um - (float)im, fp = up - (float)ip;
@@ -531,8 +594,14 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
         if constexpr (MODE == MODE_ADJ) {
           if constexpr (AA) {
             // transpose of the double integration: the same symmetric operator
-            for (int b = wave; b < nb; b += kWaves)
+            // (fixed-point windows are first converted, exactly up to 2^-53)
+            for (int b = wave; b < nb; b += kWaves) {
+              if constexpr (FX) {
+                const int off = S.b_off[b], len = S.q_len[qs + b];
+                for (int s = lane; s < len; s += 64) Wa[off + s] = from_fixed64(Wl[off + s]);
+              }
               integrate_window(Wa + S.b_off[b], nullptr, S.q_len[qs + b], lane);
+            }
             __syncthreads();
           }
           // flush the windows into the output traces
@@ -543,7 +612,10 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
             float* row = a.traces_out + (int64_t)id * a.n_samples;
             for (int s = lane; s < len; s += 64) {
               const int dst = w0 + s;
-              const float v = AA ? (float)Wa[off + s] : Wf[off + s];
+              float v;
+              if constexpr (FX && AA) v = (float)(Wa[off + s] * fx_inv);
+              else if constexpr (FX) v = (float)(from_fixed64(Wl[off + s]) * fx_inv);
+              else v = AA ? (float)Wa[off + s] : Wf[off + s];
               if (dst >= 0 && dst < a.n_samples && v != 0.f) atomicAdd(&row[dst], v);
             }
           }
@@ -672,13 +744,14 @@ inline int64_t tile_blocks(const TileArgs& a) {
   return (int64_t)8 * chunk * a.n_tblocks;
 }
 
-template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN, int OCC = default_occ(AA)>
+template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN, int OCC = default_occ(AA),
+          bool FIXED = false>
 int launch_one(const TileArgs& a, hipStream_t s) {
   const int64_t blocks = tile_blocks(a);
   if (blocks <= 0) return PSTM_OK;
   if (blocks > 0x7fffffff) return set_error(PSTM_ERR_INVALID, "too many tiles (%lld)", (long long)blocks);
-  hipLaunchKernelGGL((k_tile<MODE, TX, TY, DT, AA, VTI, GEN, OCC>), dim3((unsigned)blocks), dim3(kThreads), 0, s,
-                     a);
+  hipLaunchKernelGGL((k_tile<MODE, TX, TY, DT, AA, VTI, GEN, OCC, FIXED>), dim3((unsigned)blocks), dim3(kThreads), 0,
+                     s, a);
   return PSTM_OK;
 }
 
