@@ -233,7 +233,11 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   const int tx = tile / a.n_tiles_y, ty = tile % a.n_tiles_y;
 
   // ---- per-lane constants
-  const int k = tb * kTileTau + tid;
+  // The adjoint interleaves the waves over the block's times (wave w owns
+  // tau = w, w + 4, ...): neighbouring lanes are 4 samples apart, so the
+  // taps of one (trace, column) land on distinct window samples in one
+  // atomic instruction unless the moveout is below 1/4 sample per sample.
+  const int k = MODE == MODE_ADJ ? tb * kTileTau + lane * kWaves + wave : tb * kTileTau + tid;
   const bool kvalid = k < a.n_tau;
   const int kk = kvalid ? k : a.n_tau - 1;
   const float tau = a.tau0 + (float)kk * a.dtau;
@@ -300,8 +304,10 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
       S.hwave[wave] = hw;
       S.qmin[wave] = qn;
       S.qmax[wave] = qx;
-      const int ka = min(tb * kTileTau + wave * 64, a.n_tau - 1);
-      const int kb = min(tb * kTileTau + wave * 64 + 63, a.n_tau - 1);
+      const int ka = MODE == MODE_ADJ ? min(tb * kTileTau + wave, a.n_tau - 1)
+                                      : min(tb * kTileTau + wave * 64, a.n_tau - 1);
+      const int kb = MODE == MODE_ADJ ? min(tb * kTileTau + 63 * kWaves + wave, a.n_tau - 1)
+                                      : min(tb * kTileTau + wave * 64 + 63, a.n_tau - 1);
       const float ta = a.tau0 + (float)ka * a.dtau, tbv = a.tau0 + (float)kb * a.dtau;
       S.tqa[wave] = 0.25f * ta * ta;
       S.tqb[wave] = 0.25f * tbv * tbv;
