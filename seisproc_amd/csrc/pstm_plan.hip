@@ -23,13 +23,15 @@ __global__ void k_plan_header(PlanHeader* h, int64_t n_traces, int32_t ncx, int3
   }
 }
 
-// FP64 world -> FP32 local (along il, along xl) relative to the image origin.
+// FP64 world -> FP32 local (along il, along xl) relative to the image origin;
+// key = offset band * cells + midpoint cell.
 __global__ void k_plan_local(const double* __restrict__ src, const double* __restrict__ rcv,
                              int64_t n, double ox, double oy, double ux_x, double ux_y,
                              double uy_x, double uy_y, double lo_a, double hi_a, double lo_b,
                              double hi_b, double cx0, double cy0, double cw, double ch,
-                             int32_t ncx, int32_t ncy, float4* __restrict__ local,
-                             uint32_t* __restrict__ keys, int32_t* __restrict__ ids) {
+                             int32_t ncx, int32_t ncy, int32_t n_bands, double two_a,
+                             float4* __restrict__ local, uint32_t* __restrict__ keys,
+                             int32_t* __restrict__ ids) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   double sx = src[2 * j] - ox, sy = src[2 * j + 1] - oy;
@@ -37,16 +39,20 @@ __global__ void k_plan_local(const double* __restrict__ src, const double* __res
   double sa = sx * ux_x + sy * ux_y, sb = sx * uy_x + sy * uy_y;
   double ra = rx * ux_x + ry * ux_y, rb = rx * uy_x + ry * uy_y;
   double ma = 0.5 * (sa + ra), mb = 0.5 * (sb + rb);
-  uint32_t n_cells = (uint32_t)ncx * (uint32_t)ncy;
-  uint32_t key = n_cells;  // dropped
+  const uint32_t n_cells = (uint32_t)ncx * (uint32_t)ncy;
+  uint32_t key = n_cells * (uint32_t)n_bands;  // dropped
   // A trace can only contribute if its midpoint is within the aperture of
-  // some image column: |m - x| <= (h_s + h_r)/2 < aperture.
-  if (ma > lo_a && ma < hi_a && mb > lo_b && mb < hi_b) {
+  // some image column (|m - x| <= (h_s + h_r)/2 < aperture) and its offset is
+  // below twice the aperture (h_s + h_r >= offset).
+  const double off = hypot(sa - ra, sb - rb);
+  if (ma > lo_a && ma < hi_a && mb > lo_b && mb < hi_b && off < two_a) {
     int32_t cx = (int32_t)floor((ma - cx0) / cw);
     int32_t cy = (int32_t)floor((mb - cy0) / ch);
     cx = cx < 0 ? 0 : (cx >= ncx ? ncx - 1 : cx);
     cy = cy < 0 ? 0 : (cy >= ncy ? ncy - 1 : cy);
-    key = (uint32_t)cy * (uint32_t)ncx + (uint32_t)cx;
+    int32_t band = (int32_t)(off * (double)n_bands / two_a);
+    band = band >= n_bands ? n_bands - 1 : band;
+    key = (uint32_t)band * n_cells + (uint32_t)cy * (uint32_t)ncx + (uint32_t)cx;
   }
   local[j] = make_float4((float)sa, (float)sb, (float)ra, (float)rb);
   keys[j] = key;
@@ -60,7 +66,8 @@ __global__ void k_plan_gather(const float4* __restrict__ local, const int32_t* _
   geom[i] = local[ids[i]];
 }
 
-// cell_start[c] = first sorted index with key >= c, for c in [0, n_cells].
+// cell_start[c] = first sorted index with key >= c, for c in [0, n_cells]
+// (n_cells counts every (band, cell) key).
 __global__ void k_plan_cells(const uint32_t* __restrict__ keys, int64_t n, uint32_t n_cells,
                              int32_t* __restrict__ cell_start, PlanHeader* h) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -94,7 +101,7 @@ int plan_build(const pstm_params_t* p, const PlanLayout& L, const double* src_xy
   auto* geom = (float4*)(base + L.off_geom);
   auto* cells = (int32_t*)(base + L.off_cell_start);
   const int64_t n = L.n_traces;
-  const uint32_t n_cells = (uint32_t)L.ncx * (uint32_t)L.ncy;
+  const uint32_t n_cells = (uint32_t)L.ncx * (uint32_t)L.ncy * (uint32_t)L.n_bands;  // sort keys
 
   hipLaunchKernelGGL(k_plan_header, dim3(1), dim3(64), 0, s, hdr, n, L.ncx, L.ncy);
   if (n > 0) {
@@ -104,7 +111,7 @@ int plan_build(const pstm_params_t* p, const PlanLayout& L, const double* src_xy
     hipLaunchKernelGGL(k_plan_local, dim3(blocks), dim3(256), 0, s, src_xy, rcv_xy, n,
                        p->origin_x, p->origin_y, L.ux_x, L.ux_y, L.uy_x, L.uy_y, -A, W + A, -A,
                        H + A, (double)L.cell_x0, (double)L.cell_y0, (double)L.cell_w,
-                       (double)L.cell_h, L.ncx, L.ncy, local, keys_in, ids_in);
+                       (double)L.cell_h, L.ncx, L.ncy, L.n_bands, 2.0 * A, local, keys_in, ids_in);
     size_t tmp_bytes = L.sort_tmp_bytes;
     hipError_t e = hipcub::DeviceRadixSort::SortPairs(base + L.off_sort_tmp, tmp_bytes, keys_in,
                                                       keys_out, ids_in, ids_out, (int)n, 0,

@@ -647,9 +647,11 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
       __syncthreads();
     };
 
+    const int n_cells = a.ncx * a.ncy;
+    for (int band = a.band0; band < a.band1; ++band)
     for (int cy = cy0; cy <= cy1; ++cy) {
-      const int r0 = a.cell_start[cy * a.ncx + cx0];
-      const int r1 = a.cell_start[cy * a.ncx + cx1 + 1];
+      const int r0 = a.cell_start[band * n_cells + cy * a.ncx + cx0];
+      const int r1 = a.cell_start[band * n_cells + cy * a.ncx + cx1 + 1];
       for (int j0 = r0; j0 < r1; j0 += kThreads) {
         const int j = j0 + tid;
         bool pass = false;
@@ -756,8 +758,14 @@ int launch_one(const TileArgs& a, hipStream_t s) {
   const int64_t blocks = tile_blocks(a);
   if (blocks <= 0) return PSTM_OK;
   if (blocks > 0x7fffffff) return set_error(PSTM_ERR_INVALID, "too many tiles (%lld)", (long long)blocks);
-  hipLaunchKernelGGL((k_tile<MODE, TX, TY, DT, AA, VTI, GEN, OCC, FIXED>), dim3((unsigned)blocks), dim3(kThreads), 0,
-                     s, a);
+  // one launch per offset band (stream-ordered, accumulating)
+  TileArgs b = a;
+  for (int band = a.band0; band < a.band1; ++band) {
+    b.band0 = band;
+    b.band1 = band + 1;
+    hipLaunchKernelGGL((k_tile<MODE, TX, TY, DT, AA, VTI, GEN, OCC, FIXED>), dim3((unsigned)blocks), dim3(kThreads),
+                       0, s, b);
+  }
   return PSTM_OK;
 }
 
