@@ -40,6 +40,7 @@ struct TileShared {
   int32_t q_j[kQueueCap];       // candidate queue: sorted trace index
   int32_t q_w0[kQueueCap];      // first staged sample
   int32_t q_len[kQueueCap];     // staged window length
+  uint8_t q_nc[kQueueCap];      // 1: no output time of the tile reads past either end of the record
   int32_t b_base[kMaxBatch];    // window-element index of sample 0 of the trace (off - w0)
   int32_t b_off[kMaxBatch];     // window-element offset of the window
   int32_t scan[kWaves];
@@ -56,8 +57,10 @@ template <int NC, bool AA>
 struct PairShared {
   float2 h2[kMaxBatch][NC];     // squared source / receiver distances (m^2)
   float hm[kMaxBatch][NC];      // max of the two
-  int32_t act[kMaxBatch];       // bit w: the trace reaches some time of wave w
+  int32_t act[kMaxBatch];       // bit w: the trace reaches some time of wave w; bit 4 + w (forward
+                                // fast path): it passes every time and column of wave w, unclamped
   float hcol[kWaves][NC];       // per wave and column: max hlim^2 over the wave's times
+  float hcolmin[kWaves][NC];    // per wave and column: min hlim^2 over the wave's times
   float4 g[AA ? kMaxBatch : 1][AA ? NC : 1];  // antialias: scaled (s - x, r - x) components
 };
 

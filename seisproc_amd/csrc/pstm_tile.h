@@ -206,6 +206,14 @@ template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN, int OCC
 __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   constexpr int NC = TX * TY;
   constexpr bool FX = MODE == MODE_ADJ && FIXED;
+  // Forward fast path: traces that pass the masks at every time and column
+  // of a wave and whose tile times never reach past the record are summed
+  // without the mask, clamp and fold instructions (their fold is counted once
+  // per wave); the rest take the masked loop.  Both give the same sums.
+  constexpr bool FULLPATH = MODE == MODE_FWD && !GEN;
+  // Software prefetch of the next trace's distances (registers permitting:
+  // the plain forward and gradient forms).
+  constexpr bool PREFETCH = MODE != MODE_ADJ && !AA && !VTI && !GEN;
   static_assert(NC == 8 || (NC == 32 && MODE == MODE_ADJ), "columns per tile: 8 (32 for the adjoint)");
   static_assert(!FIXED || MODE == MODE_ADJ, "fixed-point windows are an adjoint form");
   constexpr int CAPF = sample_cap_floats<NC, AA, OCC>();         // staged-window floats
@@ -291,11 +299,15 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
     }
     hmax_l = fmaxf(hmax_l, hl2[c]);
   }
-  // per-wave column reach (max over the wave's times)
+  // per-wave column reach (max and min over the wave's times)
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const float hc = wave_max(hl2[c]);
     if (lane == 0) P.hcol[wave][c] = hc;
+    if constexpr (FULLPATH) {
+      const float hn = wave_min(hl2[c]);
+      if (lane == 0) P.hcolmin[wave][c] = hn;
+    }
   }
   {
     const float hw = wave_max(hmax_l), qn = wave_min(qmin_l), qx = wave_max(qmax_l);
@@ -327,14 +339,15 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
 
   // ---- accumulators
   float acc[NC], accb[NC], cmp[NC];
-  int fcnt[NC];
+  float fcnt[NC];  // fold counts (exact in FP32 below 2^24)
+  int nfull = 0;  // traces summed by the unmasked loop (they count at every column and time of the wave)
   [[maybe_unused]] float m_in[MODE == MODE_ADJ ? NC : 1];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     acc[c] = 0.f;
     accb[c] = 0.f;
     cmp[c] = 0.f;
-    fcnt[c] = 0;
+    fcnt[c] = 0.f;
   }
   if constexpr (MODE == MODE_ADJ) {
 #pragma unroll
@@ -426,12 +439,35 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
             const int id = a.ids[S.q_j[e]];
             const float sc = a.scale ? a.scale[id] : 1.f;
             const int64_t row = (int64_t)id * a.n_samples;
-            for (int s = lane; s < len; s += 64) {
-              const int src = w0 + s;
-              float v = 0.f;
-              if (src >= 0 && src < a.n_samples) v = load_sample<DT>(a.traces, row + src) * sc;
-              if constexpr (AA) Wa[off + s] = (double)v;
-              else Wf[off + s] = v;
+            // in-record part [lo, hi) of the window; antialias windows also
+            // hold zero padding outside the record
+            const int lo = max(w0, 0), hi = min(w0 + len, a.n_samples);
+            if (lo > w0 || hi < w0 + len) {
+              for (int s = lane; s < len; s += 64) {
+                const int src = w0 + s;
+                if (src < lo || src >= hi) {
+                  if constexpr (AA) Wa[off + s] = 0.0;
+                  else Wf[off + s] = 0.f;
+                }
+              }
+            }
+            // four independent loads per lane in flight (indexes clamped into
+            // the record, so every load is in bounds), then convert and store
+            for (int s0 = lo - w0; s0 < hi - w0; s0 += 4 * 64) {
+              float v[4];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const int src = min(w0 + s0 + lane + 64 * u, hi - 1);
+                v[u] = load_sample<DT>(a.traces, row + src);
+              }
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const int s = s0 + lane + 64 * u;
+                if (w0 + s < hi) {
+                  if constexpr (AA) Wa[off + s] = (double)(v[u] * sc);
+                  else Wf[off + s] = v[u] * sc;
+                }
+              }
             }
           }
         }
@@ -447,10 +483,15 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           P.h2[b][c] = make_float2(hs, hr);
           P.hm[b][c] = hm;
           if constexpr (AA) P.g[b][c] = make_float4(dxs * a.d_il, dys * a.d_xl, dxr * a.d_il, dyr * a.d_xl);
-          int bits = 0;
+          int bits = 0, nfb = 0;
 #pragma unroll
-          for (int w = 0; w < kWaves; ++w) bits |= hm < P.hcol[w][c] ? (1 << w) : 0;
+          for (int w = 0; w < kWaves; ++w) {
+            bits |= hm < P.hcol[w][c] ? (1 << w) : 0;
+            if constexpr (FULLPATH) nfb |= hm < P.hcolmin[w][c] ? 0 : (1 << w);
+          }
           bits = group_or<NC>(bits);
+          if constexpr (FULLPATH) nfb = group_or<NC>(nfb);  // a wave some column does not fully pass
+          if constexpr (FULLPATH) bits |= S.q_nc[qs + b] ? ((~nfb & 0xf) << 4) : 0;
           if (c == NC - 1) P.act[b] = bits;
         }
         __syncthreads();
@@ -462,10 +503,9 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
         }
 
         // -- contributions of the traces that reach this wave's samples
-        const uint64_t todo = __ballot(lane < nb && ((P.act[lane] >> wave) & 1));
-        for (uint64_t m = todo; m != 0; m &= m - 1) {
-          const int b = uniform_i(__builtin_ctzll(m));
-          const int base = uniform_i(S.b_base[b]);
+        auto trace_body = [&](const int b, const int base, const float2 (&h2v)[NC], const float (&hmv)[NC],
+                              auto full_tag) {
+          constexpr bool FULL = decltype(full_tag)::value;
           // window of the trace, indexed by sample
           [[maybe_unused]] float* const Fb = Wf + base;
           [[maybe_unused]] double* const Ab = Wa + base;
@@ -473,9 +513,17 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           [[maybe_unused]] unsigned long long* const Lb = reinterpret_cast<unsigned long long*>(Wl + base);
 #pragma unroll
           for (int c = 0; c < NC; ++c) {
-            const float2 h2 = P.h2[b][c];
-            const float hm = P.hm[b][c];
-            bool ok = hm < hl2[c];
+            const float2 h2 = h2v[c];
+            const float hm = hmv[c];
+            bool ok = FULL || hm < hl2[c];
+            // FP32 form of the same predicate for the branch-free forward:
+            // sat(2^64 (hlim^2 - hm)) is 1 exactly when hm < hlim^2 (the FMA
+            // rounds the exact, power-of-two scaled difference once, so its
+            // sign is exact; any positive difference of these magnitudes,
+            // hlim^2 >= (1e-6 tan(theta))^2, scales to >= 1).
+            [[maybe_unused]] float mk = 1.f;
+            if constexpr (MODE == MODE_FWD && !GEN && !FULL)
+              mk = __builtin_amdgcn_fmed3f(fmaf(hm, -0x1p64f, hl2[c] * 0x1p64f), 0.f, 1.f);
             const float xs = fmaf(h2.x, qd[c], tqs), xr = fmaf(h2.y, qd[c], tqs);
             const float rs = __builtin_amdgcn_rsqf(xs), rr = __builtin_amdgcn_rsqf(xr);
             float u_raw;
@@ -490,7 +538,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               u_raw = fmaf(xs, rs, fmaf(xr, rr, -a.t0_over_dt));
             }
             if (zero_out) ok = ok && (u_raw >= 0.f) && (u_raw <= a.u_last);
-            const float u = __builtin_amdgcn_fmed3f(u_raw, 0.f, a.umax);
+            const float u = FULL ? u_raw : __builtin_amdgcn_fmed3f(u_raw, 0.f, a.umax);
             const float prs = rs * rr;
             float wd = prs * prs;
             if constexpr (GEN) {
@@ -524,9 +572,14 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
                 a1 = Fb[i0 + 1];
                 amp = fmaf(f, a1 - a0, a0);
               }
-              if constexpr (MODE == MODE_FWD) {
+              if constexpr (MODE == MODE_FWD && FULL) {
+                accb[c] = fmaf(wd, amp, accb[c]);
+              } else if constexpr (MODE == MODE_FWD && !GEN) {
+                accb[c] = fmaf(wd * mk, amp, accb[c]);
+                fcnt[c] += mk;
+              } else if constexpr (MODE == MODE_FWD) {
                 accb[c] = fmaf(ok ? wd : 0.f, amp, accb[c]);
-                fcnt[c] += ok ? 1 : 0;
+                fcnt[c] += ok ? 1.f : 0.f;
               } else {
                 // d/dv of cw * wd * amp with masks held (isotropic, no antialias)
                 const float Q = qd[c] * fmaf(h2.x, rs * rs, h2.y * (rr * rr));  // q (h_s^2/t_s^2 + h_r^2/t_r^2)
@@ -580,7 +633,66 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               }
             }
           }
+        };
+        // Walk the set bits of a trace mask; the next trace's distances are
+        // read from LDS before the current trace is summed, so their latency
+        // overlaps the arithmetic.
+        auto walk = [&](uint64_t m, auto full_tag) {
+          if constexpr (!PREFETCH) {
+            for (; m != 0; m &= m - 1) {
+              const int b = uniform_i(__builtin_ctzll(m));
+              float2 h2c[NC];
+              float hmc[NC];
+#pragma unroll
+              for (int c = 0; c < NC; ++c) {
+                h2c[c] = P.h2[b][c];
+                hmc[c] = P.hm[b][c];
+              }
+              trace_body(b, uniform_i(S.b_base[b]), h2c, hmc, full_tag);
+            }
+            return;
+          }
+          if (m == 0) return;
+          int b = uniform_i(__builtin_ctzll(m));
+          float2 h2c[NC];
+          float hmc[NC];
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            h2c[c] = P.h2[b][c];
+            hmc[c] = P.hm[b][c];
+          }
+          int basec = uniform_i(S.b_base[b]);
+          while (true) {
+            m &= m - 1;
+            const bool more = m != 0;
+            const int bn = more ? uniform_i(__builtin_ctzll(m)) : b;
+            float2 h2n[NC];
+            float hmn[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+              h2n[c] = P.h2[bn][c];
+              hmn[c] = P.hm[bn][c];
+            }
+            const int basen = uniform_i(S.b_base[bn]);
+            trace_body(b, basec, h2c, hmc, full_tag);
+            if (!more) break;
+            b = bn;
+            basec = basen;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+              h2c[c] = h2n[c];
+              hmc[c] = hmn[c];
+            }
+          }
+        };
+        const uint64_t todo = __ballot(lane < nb && ((P.act[lane] >> wave) & 1));
+        uint64_t tfull = 0;
+        if constexpr (FULLPATH) {
+          tfull = __ballot(lane < nb && ((P.act[lane] >> (4 + wave)) & 1));
+          nfull += (int)__popcll(tfull);
+          walk(tfull, std::true_type{});
         }
+        walk(todo & ~tfull, std::false_type{});
         if constexpr (MODE != MODE_ADJ) {
           if (a.flags & PSTM_FLAG_KAHAN) {
 #pragma unroll
@@ -631,17 +743,19 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
       }
       // move the unprocessed tail (< kMaxBatch entries) to the front
       const int rem = qn - qs;
-      int j = 0, w0 = 0, len = 0;
+      int j = 0, w0 = 0, len = 0, nc = 0;
       if (tid < rem) {
         j = S.q_j[qs + tid];
         w0 = S.q_w0[qs + tid];
         len = S.q_len[qs + tid];
+        nc = S.q_nc[qs + tid];
       }
       __syncthreads();
       if (tid < rem) {
         S.q_j[tid] = j;
         S.q_w0[tid] = w0;
         S.q_len[tid] = len;
+        S.q_nc[tid] = (uint8_t)nc;
       }
       if (tid == 0) S.queue_n = rem > 0 ? rem : 0;
       __syncthreads();
@@ -654,7 +768,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
       const int r1 = a.cell_start[band * n_cells + cy * a.ncx + cx1 + 1];
       for (int j0 = r0; j0 < r1; j0 += kThreads) {
         const int j = j0 + tid;
-        bool pass = false;
+        bool pass = false, nclamp = false;
         int w0 = 0, len = 0;
         if (j < r1) {
           const float4 g = a.geom[j];
@@ -680,8 +794,12 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               }
             }
             if (thi >= 0.f) {
-              const float ul = __builtin_amdgcn_fmed3f(fmaf(tlo, a.inv_dt, -a.t0_over_dt), 0.f, a.umax);
-              const float uh = __builtin_amdgcn_fmed3f(fmaf(thi, a.inv_dt, -a.t0_over_dt), 0.f, a.umax);
+              const float ul_raw = fmaf(tlo, a.inv_dt, -a.t0_over_dt), uh_raw = fmaf(thi, a.inv_dt, -a.t0_over_dt);
+              const float ul = __builtin_amdgcn_fmed3f(ul_raw, 0.f, a.umax);
+              const float uh = __builtin_amdgcn_fmed3f(uh_raw, 0.f, a.umax);
+              // the staged window already trusts these bounds to kWindowMargin samples
+              nclamp = uh_raw + (float)kWindowMargin < a.umax &&
+                       (a.t0_over_dt == 0.f || ul_raw - (float)kWindowMargin > 0.f);
               int lo_i = (int)ul - pad - kWindowMargin;
               int hi_i = (int)uh + 1 + pad + kWindowMargin;
               lo_i = max(lo_i, AA ? wlo : 0);
@@ -710,6 +828,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           S.q_j[slot] = j;
           S.q_w0[slot] = w0;
           S.q_len[slot] = len;
+          S.q_nc[slot] = nclamp ? 1 : 0;
         }
         __syncthreads();
         if (tid == 0) S.queue_n = qn + tot;
@@ -736,7 +855,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
       } else {
         a.image[idx] += acc[c] + cmp[c];
       }
-      if (a.fold) a.fold[idx] += (float)fcnt[c];
+      if (a.fold) a.fold[idx] += fcnt[c] + (float)nfull;
     } else if constexpr (MODE == MODE_GRAD) {
       a.vel_grad[idx] += (acc[c] + cmp[c]) * a.image_in[idx];
     }
