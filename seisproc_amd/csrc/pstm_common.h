@@ -40,7 +40,7 @@ struct TileShared {
   int32_t q_j[kQueueCap];       // candidate queue: sorted trace index
   int32_t q_w0[kQueueCap];      // first staged sample
   int32_t q_len[kQueueCap];     // staged window length
-  uint8_t q_nc[kQueueCap];      // 1: no output time of the tile reads past either end of the record
+  uint8_t q_nc[kQueueCap];      // bit w: no time of wave w reads past either end of the record
   int32_t b_base[kMaxBatch];    // window-element index of sample 0 of the trace (off - w0)
   int32_t b_off[kMaxBatch];     // window-element offset of the window
   int32_t scan[kWaves];

@@ -491,7 +491,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           }
           bits = group_or<NC>(bits);
           if constexpr (FULLPATH) nfb = group_or<NC>(nfb);  // a wave some column does not fully pass
-          if constexpr (FULLPATH) bits |= S.q_nc[qs + b] ? ((~nfb & 0xf) << 4) : 0;
+          if constexpr (FULLPATH) bits |= (S.q_nc[qs + b] & ~nfb & 0xf) << 4;
           if (c == NC - 1) P.act[b] = bits;
         }
         __syncthreads();
@@ -768,7 +768,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
       const int r1 = a.cell_start[band * n_cells + cy * a.ncx + cx1 + 1];
       for (int j0 = r0; j0 < r1; j0 += kThreads) {
         const int j = j0 + tid;
-        bool pass = false, nclamp = false;
+        bool pass = false;
+        int nclamp = 0;  // bit w: wave w reads inside the record only
         int w0 = 0, len = 0;
         if (j < r1) {
           const float4 g = a.geom[j];
@@ -791,15 +792,16 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
                 const float hi = sqrtf(tbq + hs_max * S.qmax[w]) + sqrtf(tbq + hr_max * S.qmax[w]);
                 tlo = fminf(tlo, lo);
                 thi = fmaxf(thi, hi);
+                // wave w never reads past either end of the record (the staged
+                // window already trusts these bounds to kWindowMargin samples)
+                const bool in_rec = fmaf(hi, a.inv_dt, -a.t0_over_dt) + (float)kWindowMargin < a.umax &&
+                                    (a.t0_over_dt == 0.f || fmaf(lo, a.inv_dt, -a.t0_over_dt) > (float)kWindowMargin);
+                nclamp |= in_rec ? (1 << w) : 0;
               }
             }
             if (thi >= 0.f) {
-              const float ul_raw = fmaf(tlo, a.inv_dt, -a.t0_over_dt), uh_raw = fmaf(thi, a.inv_dt, -a.t0_over_dt);
-              const float ul = __builtin_amdgcn_fmed3f(ul_raw, 0.f, a.umax);
-              const float uh = __builtin_amdgcn_fmed3f(uh_raw, 0.f, a.umax);
-              // the staged window already trusts these bounds to kWindowMargin samples
-              nclamp = uh_raw + (float)kWindowMargin < a.umax &&
-                       (a.t0_over_dt == 0.f || ul_raw - (float)kWindowMargin > 0.f);
+              const float ul = __builtin_amdgcn_fmed3f(fmaf(tlo, a.inv_dt, -a.t0_over_dt), 0.f, a.umax);
+              const float uh = __builtin_amdgcn_fmed3f(fmaf(thi, a.inv_dt, -a.t0_over_dt), 0.f, a.umax);
               int lo_i = (int)ul - pad - kWindowMargin;
               int hi_i = (int)uh + 1 + pad + kWindowMargin;
               lo_i = max(lo_i, AA ? wlo : 0);
@@ -828,7 +830,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           S.q_j[slot] = j;
           S.q_w0[slot] = w0;
           S.q_len[slot] = len;
-          S.q_nc[slot] = nclamp ? 1 : 0;
+          S.q_nc[slot] = (uint8_t)nclamp;
         }
         __syncthreads();
         if (tid == 0) S.queue_n = qn + tot;
