@@ -50,6 +50,7 @@ struct TileShared {
   float qmin[kWaves], qmax[kWaves], tqa[kWaves], tqb[kWaves];
   float emax;                   // max eta over the tile (VTI)
   float gmax[kWaves];           // adjoint: per-wave bound of one contribution
+  float gmin[kWaves];           // adjoint: per-wave minimum of the per-time bounds
 };
 
 // Per sub-batch (trace, column) quantities of a tile of NC columns.
@@ -89,6 +90,10 @@ constexpr int default_occ(bool aa) { return aa ? 2 : 3; }
 constexpr int min_cap_elems(bool aa) {
   return aa ? sample_cap_elems<8, true, default_occ(true)>() : sample_cap_elems<8, false, default_occ(false)>();
 }
+// Elements of a bank-skewed adjoint window of `len` samples (one spare
+// element before the window and one before every 32nd sample).
+__host__ __device__ constexpr int skew_len(int len) { return len + (len >> 5) + 2; }
+
 // Window length (elements) of a whole trace with its padding.
 inline int64_t whole_window(const pstm_params_t* p, int aa_pad) {
   return (int64_t)p->n_samples + 2 * (aa_pad + kWindowMargin) + 2;

@@ -202,10 +202,21 @@ __device__ __forceinline__ float taper_of(float hm2, float hl2, float tw) {
 // bound G of its own contributions so that one contribution is < 2^47 units
 // and a window sample (<= 2^13 contributions of <= 4 taps) stays < 2^62; the
 // accumulation is exact, each contribution is rounded once at 2^-47 G.
-template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN, int OCC, bool FIXED>
+// Without antialias (SKEW) the taps are rounded in FP32 to integers below
+// 2^30 at a scale 2^(30 - ceil(log2 G)) and the window sample of trace
+// sample i sits at i + floor(i / 32) (one spare slot before every 32nd
+// sample, which receives the second tap of a pair that straddles the gap and
+// is added back at the flush): with the waves interleaved over the block's
+// times, lanes eight apart would otherwise hit the same LDS bank whenever the
+// moveout is one sample per output sample.
+//
+// V1 (velocity given per output time only, PSTM_VEL_1D): the velocity terms
+// are the same for every column of the tile and are held once per lane.
+template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN, int OCC, bool FIXED, bool V1>
 __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   constexpr int NC = TX * TY;
   constexpr bool FX = MODE == MODE_ADJ && FIXED;
+  constexpr bool SKEW = FX && !AA;  // FP32 taps rounded to integers, bank-skewed windows
   // Forward fast path: traces that pass the masks at every time and column
   // of a wave and whose tile times never reach past the record are summed
   // without the mask, clamp and fold instructions (their fold is counted once
@@ -268,11 +279,13 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
     const int il = tx * TX + c / TY, xl = ty * TY + c % TY;
     const bool cv = il < a.n_il && xl < a.n_xl;
     cvalid |= cv ? (1u << c) : 0u;
-    const int64_t vi = a.vel_mode == PSTM_VEL_1D
+    const int64_t vi = (V1 || a.vel_mode == PSTM_VEL_1D)
                            ? (int64_t)kk
                            : ((int64_t)(cv ? il : 0) * a.n_xl + (cv ? xl : 0)) * a.n_tau + kk;
     const float v = a.vel[vi];
-    const bool ok = kvalid && cv && v > 0.f;
+    // with V1 the columns outside the image keep the common values (their
+    // sums are never written and their adjoint input is zero)
+    const bool ok = kvalid && (V1 || cv) && v > 0.f;
     const float vv = ok ? v : 1.f;
     const float q = 1.0f / (vv * vv);
     qd[c] = q * idt2;
@@ -358,6 +371,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
     }
   }
   [[maybe_unused]] double fx_scale = 0.0, fx_inv = 0.0;
+  bool narrow = false;  // SKEW: taps in FP32 at 2^(30 - e) (else FP64 at 2^(47 - e))
   if constexpr (FX) {
     // bound of one contribution: |m cw| times the largest weight power
     // (rs, rr <= 1/sqrt(tqs)), times 4 for the antialias taps
@@ -366,18 +380,36 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
     float gl = 0.f;
 #pragma unroll
     for (int c = 0; c < NC; ++c) gl = fmaxf(gl, fabsf(m_in[c]) * wdmax);
-    gl = wave_max(AA ? 4.f * gl : gl);
-    if (lane == 0) S.gmax[wave] = gl;
+    if (AA) gl *= 4.f;
+    const float gw = wave_max(gl), gn = wave_min(gl > 0.f ? gl : 3.0e38f);
+    if (lane == 0) {
+      S.gmax[wave] = gw;
+      S.gmin[wave] = gn;
+    }
     __syncthreads();
-    float G = 0.f;
+    float G = 0.f, Gmin = 3.0e38f;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) G = fmaxf(G, S.gmax[w]);
+    for (int w = 0; w < kWaves; ++w) {
+      G = fmaxf(G, S.gmax[w]);
+      Gmin = fminf(Gmin, S.gmin[w]);
+    }
     if (!(G > 0.f)) return;  // an all-zero image tile scatters nothing (uniform over the workgroup)
     G = fminf(G, 3.0e38f);
     int e;
     (void)frexpf(G, &e);  // G < 2^e
-    fx_scale = ldexp(1.0, 47 - e);
-    fx_inv = ldexp(1.0, e - 47);
+    // FP32 taps rounded at 2^-30 G when the per-time bounds span at most a
+    // factor 16 (every block but the shallowest under the FULL weight); else
+    // FP64 taps rounded at 2^-47 G
+    narrow = SKEW && G <= 16.f * Gmin;
+    if (narrow) {
+      // taps |w m| 2^(30 - e) < 2^30: the image values carry the scale
+#pragma unroll
+      for (int c = 0; c < NC; ++c) m_in[c] = ldexpf(m_in[c], 30 - e);
+      fx_inv = ldexp(1.0, e - 30);
+    } else {
+      fx_scale = ldexp(1.0, 47 - e);
+      fx_inv = ldexp(1.0, e - 47);
+    }
   }
 
   const bool use_taper = GEN ? (a.flags & PSTM_FLAG_TAPER) != 0 : false;
@@ -411,14 +443,16 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
         if (wave == 0) {
           const int i = qs + lane;
           const bool in = lane < kMaxBatch && i < qn;
-          const int cost = in ? S.q_len[i] : 0;  // window elements
+          const int cost = in ? (SKEW ? skew_len(S.q_len[i]) : S.q_len[i]) : 0;  // window elements
           const int incl = wave_scan(cost, lane);
           const bool fit = in && incl <= CAPE;
           const uint64_t m = __ballot(fit);
           if (fit) {
             const int off = incl - cost;
+            const int w0 = S.q_w0[i];
             S.b_off[lane] = off;
-            S.b_base[lane] = off - S.q_w0[i];
+            // SKEW: sample i at base + i + (i >> 5), one spare slot before the window (w0 >= 0)
+            S.b_base[lane] = SKEW ? off + 1 - (w0 + (w0 >> 5)) : off - w0;
           }
           if (lane == 0) S.nb = __popcll(m);
         }
@@ -430,7 +464,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           const int e = qs + b;
           const int w0 = S.q_w0[e], len = S.q_len[e], off = S.b_off[b];
           if constexpr (MODE == MODE_ADJ) {
-            for (int s = lane; s < len; s += 64) {
+            const int span = SKEW ? skew_len(len) : len;
+            for (int s = lane; s < span; s += 64) {
               if constexpr (FX) Wl[off + s] = 0;
               else if constexpr (AA) Wa[off + s] = 0.0;
               else Wf[off + s] = 0.f;
@@ -492,6 +527,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           bits = group_or<NC>(bits);
           if constexpr (FULLPATH) nfb = group_or<NC>(nfb);  // a wave some column does not fully pass
           if constexpr (FULLPATH) bits |= (S.q_nc[qs + b] & ~nfb & 0xf) << 4;
+          else if constexpr (MODE == MODE_ADJ && !AA) bits |= (S.q_nc[qs + b] & 0xf) << 4;
           if (c == NC - 1) P.act[b] = bits;
         }
         __syncthreads();
@@ -504,8 +540,17 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
 
         // -- contributions of the traces that reach this wave's samples
         auto trace_body = [&](const int b, const int base, const float2 (&h2v)[NC], const float (&hmv)[NC],
-                              auto full_tag) {
+                              auto full_tag, auto narrow_tag) {
+          // FULL: the trace's times stay inside the record in this wave (no
+          // clamp); in the forward it also passes every mask (UNMASKED)
           constexpr bool FULL = decltype(full_tag)::value;
+          constexpr bool UNMASKED = FULL && MODE == MODE_FWD;
+          constexpr bool NARROW = decltype(narrow_tag)::value;
+          // adjoint, clamping traces: the contributions read past the record
+          // end all land on samples n-2 and n-1; they are summed in a register
+          // and scattered once per trace (same-address LDS atomics of many
+          // lanes serialise)
+          [[maybe_unused]] float cacc = 0.f;
           // window of the trace, indexed by sample
           [[maybe_unused]] float* const Fb = Wf + base;
           [[maybe_unused]] double* const Ab = Wa + base;
@@ -515,7 +560,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           for (int c = 0; c < NC; ++c) {
             const float2 h2 = h2v[c];
             const float hm = hmv[c];
-            bool ok = FULL || hm < hl2[c];
+            bool ok = UNMASKED || hm < hl2[c];
             // FP32 form of the same predicate for the branch-free forward:
             // sat(2^64 (hlim^2 - hm)) is 1 exactly when hm < hlim^2 (the FMA
             // rounds the exact, power-of-two scaled difference once, so its
@@ -596,9 +641,25 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               }
             } else {
               // adjoint: scatter w * m into the window (transpose of the taps)
-              const float gsc = ok ? wd * m_in[c] : 0.f;
+              float gsc = ok ? wd * m_in[c] : 0.f;
+              if constexpr (!AA && !FULL) {
+                const bool cl = u_raw > a.umax;
+                cacc += cl ? gsc : 0.f;
+                gsc = cl ? 0.f : gsc;
+              }
               if (gsc != 0.f) {
-                if constexpr (FX) {
+                if constexpr (SKEW && NARROW) {
+                  // gsc < 2^30 in scaled units; each tap rounded toward zero
+                  const int pos = i0 + (i0 >> 5);
+                  const float t1 = gsc * f, t0 = gsc - t1;
+                  atomicAdd(&Lb[pos], (unsigned long long)(long long)(int)t0);
+                  atomicAdd(&Lb[pos + 1], (unsigned long long)(long long)(int)t1);
+                } else if constexpr (SKEW) {
+                  const int pos = i0 + (i0 >> 5);
+                  const double g = (double)gsc * fx_scale, gf = g * (double)f;
+                  atomicAdd(&Lb[pos], to_fixed64(g - gf));
+                  atomicAdd(&Lb[pos + 1], to_fixed64(gf));
+                } else if constexpr (FX) {
                   const double g = (double)gsc * fx_scale, gf = g * (double)f;
                   if constexpr (AA) {
                     const float um = u - L, up = u + L;
@@ -633,11 +694,30 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               }
             }
           }
+          if constexpr (MODE == MODE_ADJ && !AA && !FULL) {
+            if (__ballot(cacc != 0.f) != 0) {
+              const float tot = wave_scan(cacc, lane);  // lane 63: the wave's sum
+              if (lane == 63 && tot != 0.f) {
+                const int ic = (int)a.umax;  // clamped position: sample n-2, fraction 0.999
+                const float fc = __builtin_amdgcn_fractf(a.umax);
+                const float t1 = tot * fc, t0 = tot - t1;
+                if constexpr (FX) {
+                  const double sc = NARROW ? 1.0 : fx_scale;  // narrow: already in scaled units
+                  const int pos = SKEW ? ic + (ic >> 5) : ic;
+                  atomicAdd(&Lb[pos], to_fixed64((double)t0 * sc));
+                  atomicAdd(&Lb[pos + 1], to_fixed64((double)t1 * sc));
+                } else {
+                  atomicAdd(&Fb[ic], t0);
+                  atomicAdd(&Fb[ic + 1], t1);
+                }
+              }
+            }
+          }
         };
         // Walk the set bits of a trace mask; the next trace's distances are
         // read from LDS before the current trace is summed, so their latency
         // overlaps the arithmetic.
-        auto walk = [&](uint64_t m, auto full_tag) {
+        auto walk = [&](uint64_t m, auto full_tag, auto narrow_tag) {
           if constexpr (!PREFETCH) {
             for (; m != 0; m &= m - 1) {
               const int b = uniform_i(__builtin_ctzll(m));
@@ -648,7 +728,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
                 h2c[c] = P.h2[b][c];
                 hmc[c] = P.hm[b][c];
               }
-              trace_body(b, uniform_i(S.b_base[b]), h2c, hmc, full_tag);
+              trace_body(b, uniform_i(S.b_base[b]), h2c, hmc, full_tag, narrow_tag);
             }
             return;
           }
@@ -674,7 +754,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               hmn[c] = P.hm[bn][c];
             }
             const int basen = uniform_i(S.b_base[bn]);
-            trace_body(b, basec, h2c, hmc, full_tag);
+            trace_body(b, basec, h2c, hmc, full_tag, narrow_tag);
             if (!more) break;
             b = bn;
             basec = basen;
@@ -690,9 +770,21 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
         if constexpr (FULLPATH) {
           tfull = __ballot(lane < nb && ((P.act[lane] >> (4 + wave)) & 1));
           nfull += (int)__popcll(tfull);
-          walk(tfull, std::true_type{});
+          walk(tfull, std::true_type{}, std::true_type{});
         }
-        walk(todo & ~tfull, std::false_type{});
+        if constexpr (MODE == MODE_ADJ && !AA) {
+          // traces inside the record in this wave, then those that clamp
+          tfull = __ballot(lane < nb && ((P.act[lane] >> (4 + wave)) & 1));
+          if (!SKEW || narrow) {
+            walk(tfull, std::true_type{}, std::true_type{});
+            walk(todo & ~tfull, std::false_type{}, std::true_type{});
+          } else {
+            walk(tfull, std::true_type{}, std::false_type{});
+            walk(todo & ~tfull, std::false_type{}, std::false_type{});
+          }
+        } else {
+          walk(todo & ~tfull, std::false_type{}, std::true_type{});
+        }
         if constexpr (MODE != MODE_ADJ) {
           if (a.flags & PSTM_FLAG_KAHAN) {
 #pragma unroll
@@ -728,10 +820,15 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
             const int w0 = S.q_w0[e], len = S.q_len[e], off = S.b_off[b];
             const int id = a.ids[S.q_j[e]];
             float* row = a.traces_out + (int64_t)id * a.n_samples;
+            [[maybe_unused]] const long long* const Lw = Wl + S.b_base[b];
             for (int s = lane; s < len; s += 64) {
               const int dst = w0 + s;
               float v;
-              if constexpr (FX && AA) v = (float)(Wa[off + s] * fx_inv);
+              if constexpr (SKEW) {
+                const int pos = dst + (dst >> 5);
+                const long long t = Lw[pos] + ((dst & 31) == 0 ? Lw[pos - 1] : 0ll);
+                v = (float)(from_fixed64(t) * fx_inv);
+              } else if constexpr (FX && AA) v = (float)(Wa[off + s] * fx_inv);
               else if constexpr (FX) v = (float)(from_fixed64(Wl[off + s]) * fx_inv);
               else v = AA ? (float)Wa[off + s] : Wf[off + s];
               if (dst >= 0 && dst < a.n_samples && v != 0.f) atomicAdd(&row[dst], v);
@@ -874,7 +971,7 @@ inline int64_t tile_blocks(const TileArgs& a) {
 }
 
 template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN, int OCC = default_occ(AA),
-          bool FIXED = false>
+          bool FIXED = false, bool V1 = false>
 int launch_one(const TileArgs& a, hipStream_t s) {
   const int64_t blocks = tile_blocks(a);
   if (blocks <= 0) return PSTM_OK;
@@ -884,7 +981,7 @@ int launch_one(const TileArgs& a, hipStream_t s) {
   for (int band = a.band0; band < a.band1; ++band) {
     b.band0 = band;
     b.band1 = band + 1;
-    hipLaunchKernelGGL((k_tile<MODE, TX, TY, DT, AA, VTI, GEN, OCC, FIXED>), dim3((unsigned)blocks), dim3(kThreads),
+    hipLaunchKernelGGL((k_tile<MODE, TX, TY, DT, AA, VTI, GEN, OCC, FIXED, V1>), dim3((unsigned)blocks), dim3(kThreads),
                        0, s, b);
   }
   return PSTM_OK;
@@ -921,6 +1018,8 @@ int launch_forward_impl(const pstm_params_t* p, const PlanLayout& L, const TileA
     return vti ? launch_one<MODE_FWD, 4, 2, DT, true, true, false, 3>(a, s)
                : launch_one<MODE_FWD, 4, 2, DT, true, false, false, 3>(a, s);
   }
+  if (a.tile_x == 4 && a.tile_y == 2 && !gen && !aa && !vti && p->vel_mode == PSTM_VEL_1D)
+    return launch_one<MODE_FWD, 4, 2, DT, false, false, false, default_occ(false), false, true>(a, s);
   if (a.tile_x == 4 && a.tile_y == 2) return gen ? PSTM_FWD4(4, 2, true) : PSTM_FWD4(4, 2, false);
   if (a.tile_x == 8 && a.tile_y == 1) return PSTM_FWD4(8, 1, true);
   if (a.tile_x == 1 && a.tile_y == 8) return PSTM_FWD4(1, 8, true);
