@@ -1018,8 +1018,6 @@ int launch_forward_impl(const pstm_params_t* p, const PlanLayout& L, const TileA
     return vti ? launch_one<MODE_FWD, 4, 2, DT, true, true, false, 3>(a, s)
                : launch_one<MODE_FWD, 4, 2, DT, true, false, false, 3>(a, s);
   }
-  if (a.tile_x == 4 && a.tile_y == 2 && !gen && !aa && !vti && p->vel_mode == PSTM_VEL_1D)
-    return launch_one<MODE_FWD, 4, 2, DT, false, false, false, default_occ(false), false, true>(a, s);
   if (a.tile_x == 4 && a.tile_y == 2) return gen ? PSTM_FWD4(4, 2, true) : PSTM_FWD4(4, 2, false);
   if (a.tile_x == 8 && a.tile_y == 1) return PSTM_FWD4(8, 1, true);
   if (a.tile_x == 1 && a.tile_y == 8) return PSTM_FWD4(1, 8, true);
