@@ -1,6 +1,7 @@
 """PSTM throughput benchmark on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg3|cfg2|cfg5|cfg4]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--workload pstm3d-precision|pstm3d-single-offset|pstm3d-vti-largest|lsm3d]
 
 Default workload (the metric's precision form): 3D PSTM of 25 offset classes
 x 40,000 traces (1,000,000 traces x 1001 samples at 2 ms) into a 400 x 400 x
@@ -18,7 +19,7 @@ The CPU baseline is the torch-CPU restatement of the reference's CPU method
 100-point reference tile (10 x 10 image columns) at every 16th output time,
 with every trace that reaches it.
 
-``--workload cfg4`` measures least-squares migration instead: a step is one
+``--workload lsm3d`` measures least-squares migration instead: a step is one
 CG iteration (one adjoint + one forward launch over 1,048,576 traces x 751
 samples and a 256 x 256 x 751 image); the rate counts both launches'
 contributions, the CPU baseline times the restated forward and its transpose
@@ -53,8 +54,10 @@ def _parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cfg3", choices=["cfg3", "cfg2", "cfg5", "cfg4"])
-    ap.add_argument("--classes", type=int, default=None, help="offset classes (cfg3: 25)")
+    ap.add_argument("--workload", default="pstm3d-precision",
+                    choices=["pstm3d-precision", "pstm3d-single-offset", "pstm3d-vti-largest", "lsm3d"])
+    ap.add_argument("--classes", type=int, default=None,
+                    help="offset classes (pstm3d-precision: 25, pstm3d-vti-largest: 60)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--cpu-tau-stride", type=int, default=16)
@@ -67,17 +70,17 @@ def _parse():
 def _workload(args, torch):
     from seisproc_amd import synthetic as S
     dev = "cuda"
-    if args.workload == "cfg3":
+    if args.workload == "pstm3d-precision":
         ncls = args.classes or 25
-        s = S.config3_precision(n_classes=ncls, device=dev)
+        s = S.precision_3d(n_classes=ncls, device=dev)
         cfg = {"workload": "pstm3d-precision", "offset_classes": ncls, "traces": s.n_traces,
                "samples": s.params.n_samples, "dt_ms": 2.0,
                "image": [s.params.n_il, s.params.n_xl, s.params.n_tau], "bin_m": 12.5,
                "aperture_m": 1500.0, "max_angle_deg": 60.0, "antialias": False, "kahan": True,
                "velocity": "v_rms = 2000 m/s"}
         forms = ["bf16", "fp32", "fp16"]
-    elif args.workload == "cfg2":
-        s = S.config2_3d(device=dev)
+    elif args.workload == "pstm3d-single-offset":
+        s = S.single_offset_3d(device=dev)
         cfg = {"workload": "pstm3d-single-offset", "traces": s.n_traces, "samples": s.params.n_samples,
                "image": [s.params.n_il, s.params.n_xl, s.params.n_tau], "bin_m": 25.0,
                "aperture_m": 3000.0, "antialias": True,
@@ -85,7 +88,7 @@ def _workload(args, torch):
         forms = ["fp32", "bf16"]
     else:
         # --classes n: n classes over the same 0-4000 m offset range (a launch must end within minutes)
-        s = S.config5_vti(device=dev, n_classes=args.classes or 60)
+        s = S.vti_largest_3d(device=dev, n_classes=args.classes or 60)
         cfg = {"workload": "pstm3d-vti-largest", "traces": s.n_traces, "classes": len(s.class_sizes),
                "samples": s.params.n_samples, "image": [s.params.n_il, s.params.n_xl, s.params.n_tau],
                "bin_m": 12.5, "aperture_m": 4000.0, "antialias": True, "vti": True}
@@ -191,7 +194,7 @@ def _bench_lsm(args, torch):
     from seisproc_amd.processors.migration.lsm import cgls
     from seisproc_amd.processors.migration.velocity_update import image_power
     t_start = time.perf_counter()
-    s = S.config4_lsm(device="cuda")
+    s = S.lsm_3d(device="cuda")
     plan, vel, data, p = s.info["plan"], s.vel, s.traces, s.params
     _, fold = plan.forward(data, vel, with_fold=True, flags=0)
     contrib = float(fold.double().sum())
@@ -309,7 +312,7 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(1)
-    if args.workload == "cfg4":
+    if args.workload == "lsm3d":
         _bench_lsm(args, torch)
         return
     t_start = time.perf_counter()
@@ -365,8 +368,16 @@ def main():
             image.zero_()
             comp.zero_()
             fold.zero_()
+            # a step of the largest workload runs for minutes: its launches go one
+            # offset band at a time and each finished band prints a line
+            prog = None
+            if args.workload == "pstm3d-vti-largest":
+                t_s = time.perf_counter()
+                prog = (lambda f, msg, i=i, t_s=t_s: print(
+                    f"[bench] {form} step {i + 1}/{k}: {msg} ({time.perf_counter() - t_s:.1f} s)",
+                    file=sys.stderr, flush=True))
             e0.record(stream)
-            plan.forward(tr, vel, scale=sc, eta=eta, image=image, comp=comp, fold=fold)
+            plan.forward(tr, vel, scale=sc, eta=eta, image=image, comp=comp, fold=fold, progress_callback=prog)
             e1.record(stream)
             e1.synchronize()  # one line per step (long launches)
             print(f"[bench] {form} step {i + 1}/{k}: kernel {e0.elapsed_time(e1):.1f} ms", file=sys.stderr,

@@ -135,6 +135,20 @@ int pstm_forward(const pstm_params_t* p, const void* ws, const void* traces,
                  const float* trace_scale, const float* vel, const float* eta,
                  float* image, float* image_comp, float* fold, void* stream);
 
+/* Number of offset bands a plan for `p` sorts its traces into (the plan
+ * orders traces by offset band, then midpoint cell), or a negative status. */
+int pstm_plan_bands(const pstm_params_t* p);
+
+/* pstm_forward restricted to the traces of offset bands [band_begin,
+ * band_end).  Calls over disjoint band ranges accumulate into the same
+ * image, compensation and fold exactly as one pstm_forward call over their
+ * union, so a caller can report progress (the reference's progress
+ * callback, processors/migration/kirchhoff_kernel.py:760-774) or
+ * checkpoint between them. */
+int pstm_forward_bands(const pstm_params_t* p, const void* ws, int32_t band_begin, int32_t band_end,
+                       const void* traces, const float* trace_scale, const float* vel, const float* eta,
+                       float* image, float* image_comp, float* fold, void* stream);
+
 /* Adjoint of pstm_forward (Kirchhoff demigration / modelling):
  * traces_out += L^T image, FP32 [n_traces][n_samples].  Same masks, weights,
  * interpolation and antialias taps as the forward operator.

@@ -246,13 +246,32 @@ static int prep(const pstm_params_t* p, const void* ws, PlanLayout* L, TileArgs*
   return PSTM_OK;
 }
 
+int pstm_plan_bands(const pstm_params_t* p) {
+  g_last_error.clear();
+  PlanLayout L;
+  const int rc = compute_layout(p, &L);
+  return rc != PSTM_OK ? rc : L.n_bands;
+}
+
 int pstm_forward(const pstm_params_t* p, const void* ws, const void* traces,
                  const float* trace_scale, const float* vel, const float* eta, float* image,
                  float* image_comp, float* fold, void* stream) {
+  const int nb = pstm_plan_bands(p);
+  if (nb < 0) return nb;
+  return pstm_forward_bands(p, ws, 0, nb, traces, trace_scale, vel, eta, image, image_comp, fold, stream);
+}
+
+int pstm_forward_bands(const pstm_params_t* p, const void* ws, int32_t band_begin, int32_t band_end,
+                       const void* traces, const float* trace_scale, const float* vel, const float* eta,
+                       float* image, float* image_comp, float* fold, void* stream) {
   PlanLayout L;
   TileArgs a;
   int rc = prep(p, ws, &L, &a);
   if (rc != PSTM_OK) return rc;
+  if (band_begin < 0 || band_end < band_begin || band_end > L.n_bands)
+    return set_error(PSTM_ERR_INVALID, "offset bands [%d, %d) outside [0, %d)", band_begin, band_end, L.n_bands);
+  a.band0 = band_begin;
+  a.band1 = band_end;
   if (!vel || !image || (p->n_traces > 0 && !traces))
     return set_error(PSTM_ERR_INVALID, "NULL traces, vel or image");
   if ((p->flags & PSTM_FLAG_VTI) && !eta) return set_error(PSTM_ERR_INVALID, "PSTM_FLAG_VTI needs eta");

@@ -29,6 +29,9 @@ SYMBOLS = {
     "pstm_quantize_traces": (ctypes.c_int, [_V, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                             _V, _V, _V]),
     "pstm_forward": (ctypes.c_int, [_P, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
+    "pstm_plan_bands": (ctypes.c_int, [_P]),
+    "pstm_forward_bands": (ctypes.c_int, [_P, _V, ctypes.c_int32, ctypes.c_int32, _V, _V, _V, _V, _V, _V, _V,
+                                          _V]),
     "pstm_adjoint": (ctypes.c_int, [_P, _V, _V, _V, _V, _V, _V]),
     "pstm_vel_grad": (ctypes.c_int, [_P, _V, _V, _V, _V, _V, _V, _V]),
 }

@@ -14,7 +14,7 @@ missing GPU or library raises.
 from __future__ import annotations
 
 import ctypes
-from typing import Optional, Tuple
+from typing import Callable, Optional, Tuple
 
 import torch
 
@@ -118,7 +118,8 @@ class PstmPlan:
     def forward(self, traces: torch.Tensor, vel: torch.Tensor, *, scale: Optional[torch.Tensor] = None,
                 eta: Optional[torch.Tensor] = None, image: Optional[torch.Tensor] = None,
                 comp: Optional[torch.Tensor] = None, fold: Optional[torch.Tensor] = None,
-                with_fold: bool = True, flags: Optional[int] = None):
+                with_fold: bool = True, flags: Optional[int] = None,
+                progress_callback: Optional[Callable[[float, str], None]] = None):
         """Accumulate the migrated image (and fold) of ``traces``.
 
         traces: (n_traces, n_samples) float32 / bfloat16 / float16 (trace-major)
@@ -126,6 +127,11 @@ class PstmPlan:
         vel:    (n_tau,) or (n_il, n_xl, n_tau) float32 RMS velocity
         eta:    same layout as vel (PSTM_FLAG_VTI)
         image/comp/fold: optional accumulators (allocated as zeros if None).
+        progress_callback: called as (fraction, message) after each offset
+            band of the plan has been migrated (the launches are split per
+            band and the stream is synchronised between them; the sums are
+            the same as one call's), like the reference kernel's callback
+            (processors/migration/kirchhoff_kernel.py:760-774).
         Returns (image, fold) -- fold is None when with_fold is False.
         """
         p = self.params
@@ -151,10 +157,18 @@ class PstmPlan:
             fold = torch.zeros(self.image_shape, dtype=torch.float32, device=self.device)
         if fold is not None:
             _need_cuda(fold, "fold", torch.float32, self.image_shape)
-        check(self.lib.pstm_forward(ctypes.byref(cp), _ptr(self.workspace), _ptr(traces), _ptr(scale),
-                                    _ptr(vel), _ptr(eta if cp.flags & FLAG_VTI else None), _ptr(image),
-                                    _ptr(comp), _ptr(fold), ctypes.c_void_p(_stream_handle())),
-              "pstm_forward", self.lib)
+        args = (_ptr(traces), _ptr(scale), _ptr(vel), _ptr(eta if cp.flags & FLAG_VTI else None), _ptr(image),
+                _ptr(comp), _ptr(fold), ctypes.c_void_p(_stream_handle()))
+        if progress_callback is None:
+            check(self.lib.pstm_forward(ctypes.byref(cp), _ptr(self.workspace), *args), "pstm_forward", self.lib)
+            return image, fold
+        n_bands = self.lib.pstm_plan_bands(ctypes.byref(cp))
+        check(min(n_bands, 0), "pstm_plan_bands", self.lib)
+        for band in range(n_bands):
+            check(self.lib.pstm_forward_bands(ctypes.byref(cp), _ptr(self.workspace), band, band + 1, *args),
+                  "pstm_forward_bands", self.lib)
+            torch.cuda.current_stream(self.device).synchronize()
+            progress_callback((band + 1) / n_bands, f"offset band {band + 1}/{n_bands} migrated")
         return image, fold
 
     def adjoint(self, image: torch.Tensor, vel: torch.Tensor, *, eta: Optional[torch.Tensor] = None,

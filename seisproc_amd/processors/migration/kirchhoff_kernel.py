@@ -88,7 +88,10 @@ def migrate_kirchhoff_full(traces: torch.Tensor, source_x: torch.Tensor, source_
     plan = PstmPlan(p, src, rcv, device=dev)
     tr = traces.detach().to(device=dev, dtype=torch.float32).t().contiguous()
     vel = torch.full((n_tau,), float(velocity), dtype=torch.float32, device=dev)
-    img, fold = plan.forward(tr, vel)
+    band_cb = None
+    if progress_callback:
+        band_cb = lambda f, m: progress_callback(0.99 * f, m)  # noqa: E731
+    img, fold = plan.forward(tr, vel, progress_callback=band_cb)
     image = img.permute(2, 0, 1).contiguous().to(traces.device)
     fold = fold.permute(2, 0, 1).contiguous().to(traces.device)
     if enable_profiling or progress_callback:

@@ -128,7 +128,10 @@ class MigrationEngine:
         image = torch.zeros(plan.image_shape, dtype=torch.float32, device=self.device)
         comp = torch.zeros_like(image) if kahan else None
         fold = torch.zeros_like(image)
-        plan.forward(tr, vel, scale=scale, eta=eta, image=image, comp=comp, fold=fold)
+        band_cb = None
+        if progress_callback:
+            band_cb = lambda f, m: progress_callback(0.1 + 0.8 * f, m)  # noqa: E731
+        plan.forward(tr, vel, scale=scale, eta=eta, image=image, comp=comp, fold=fold, progress_callback=band_cb)
         if comp is not None:
             image += comp
         image = image.permute(2, 0, 1).contiguous()

@@ -213,7 +213,7 @@ def _image_params(n_il, n_xl, n_tau, d_il, d_xl, n_samples, dt, aperture, angle,
         xl_dx=0.0, xl_dy=d_xl, aperture_m=aperture, max_angle_deg=angle, flags=flags, **kw))
 
 
-def _events_cfg1(gen, extent_x, extent_y, t_max, n_diff=5, n_flat=3):
+def _events_line(gen, extent_x, extent_y, t_max, n_diff=5, n_flat=3):
     dx = torch.rand(n_diff, generator=gen, dtype=torch.float64)
     dy = torch.rand(n_diff, generator=gen, dtype=torch.float64)
     dtau = torch.rand(n_diff, generator=gen, dtype=torch.float64)
@@ -226,7 +226,7 @@ def _events_cfg1(gen, extent_x, extent_y, t_max, n_diff=5, n_flat=3):
     return diff, flats
 
 
-def config1_2d(seed: int = 1, n_cmp: int = 400, n_off: int = 48, n_samples: int = 1001,
+def line2d_survey(seed: int = 1, n_cmp: int = 400, n_off: int = 48, n_samples: int = 1001,
                device="cpu") -> SyntheticSurvey:
     """2D line: n_cmp CMPs at 12.5 m, offsets 0:50:(50 n_off - 50) m, 1001
     samples at 2 ms, 25 Hz Ricker at 5 diffractors + 3 flat reflectors,
@@ -240,17 +240,17 @@ def config1_2d(seed: int = 1, n_cmp: int = 400, n_off: int = 48, n_samples: int 
     oo = offs.repeat(n_cmp)
     src = torch.stack([mx - 0.5 * oo, torch.zeros_like(mx)], 1)
     rcv = torch.stack([mx + 0.5 * oo, torch.zeros_like(mx)], 1)
-    diff, flats = _events_cfg1(gen, 12.5 * (n_cmp - 1), 0.0, dt * (n_samples - 1))
+    diff, flats = _events_line(gen, 12.5 * (n_cmp - 1), 0.0, dt * (n_samples - 1))
     vf = VrmsField(v)
     tr = synthesize(src.to(device), rcv.to(device), n_samples, dt, 25.0, diff, flats, vfield=vf,
                     noise=0.05, gen=gen)
     p = _image_params(n_cmp, 1, n_samples, 12.5, 12.5, n_samples, dt, 1500.0, 60.0, FLAG_AA,
                       n_traces=src.shape[0], aa_scale=1.0, aa_max=16.0)
-    return SyntheticSurvey("cfg1-2d", p, src.to(device), rcv.to(device), tr, vf.on_grid(p, device),
+    return SyntheticSurvey("line2d", p, src.to(device), rcv.to(device), tr, vf.on_grid(p, device),
                            info=dict(diffractors=diff, flats=flats, v=v))
 
 
-def config2_3d(seed: int = 2, n_bins: int = 200, n_samples: int = 1501, n_diff: int = 200,
+def single_offset_3d(seed: int = 2, n_bins: int = 200, n_samples: int = 1501, n_diff: int = 200,
                device="cpu", aa: bool = True) -> SyntheticSurvey:
     """3D single offset class: n_bins^2 CMP bins at 25 m, offset 500 m, random
     azimuth; 1501 samples at 2 ms; v_rms = (1500 + 600 tau)(1 +- 5 % smooth
@@ -276,15 +276,15 @@ def config2_3d(seed: int = 2, n_bins: int = 200, n_samples: int = 1501, n_diff: 
     tr = synthesize(src.to(device), rcv.to(device), n_samples, dt, 30.0, diff, None, planes, vfield=vf)
     p = _image_params(n_bins, n_bins, n_samples, d, d, n_samples, dt, 3000.0, 60.0, FLAG_AA if aa else 0,
                       n_traces=src.shape[0], aa_scale=1.0, aa_max=16.0)
-    return SyntheticSurvey("cfg2-3d-fp32", p, src.to(device), rcv.to(device), tr, vf.on_grid(p, device),
+    return SyntheticSurvey("pstm3d-single-offset", p, src.to(device), rcv.to(device), tr, vf.on_grid(p, device),
                            class_sizes=sizes, info=dict(diffractors=diff, planes=planes))
 
 
-def config3_precision(seed: int = 3, n_bins: int = 200, n_classes: int = 25, n_samples: int = 1001,
+def precision_3d(seed: int = 3, n_bins: int = 200, n_classes: int = 25, n_samples: int = 1001,
                       n_img: int = 400, device="cpu") -> SyntheticSurvey:
     """Precision form: n_classes offset classes 100:100:2500 m, one trace per
     25 m CMP bin per class (n_bins^2 each, random azimuth), 1001 samples at
-    2 ms; cfg1's generator (5 diffractors + 3 flat reflectors, 25 Hz, v = 2000,
+    2 ms; the 2D line's generator (5 diffractors + 3 flat reflectors, 25 Hz, v = 2000,
     noise 0.05 of peak); image n_img^2 x 1001 at 12.5 m; aperture 1500 m, 60
     deg; compensated accumulation, no antialias."""
     gen = torch.Generator().manual_seed(seed)
@@ -292,18 +292,18 @@ def config3_precision(seed: int = 3, n_bins: int = 200, n_classes: int = 25, n_s
     ext = d * (n_bins - 1)
     offsets = [100.0 * (i + 1) for i in range(n_classes)]
     src, rcv, sizes = _cmp_grid_geometry(gen, n_bins, n_bins, d, offsets)
-    diff, flats = _events_cfg1(gen, ext, ext, dt * (n_samples - 1))
+    diff, flats = _events_line(gen, ext, ext, dt * (n_samples - 1))
     vf = VrmsField(2000.0)
     tr = synthesize(src.to(device), rcv.to(device), n_samples, dt, 25.0, diff, flats, vfield=vf,
                     noise=0.05, gen=gen)
     d_img = ext / (n_img - 1) if n_img > 1 else 12.5
     p = _image_params(n_img, n_img, n_samples, d_img, d_img, n_samples, dt, 1500.0, 60.0, FLAG_KAHAN,
                       n_traces=src.shape[0])
-    return SyntheticSurvey("cfg3-precision", p, src.to(device), rcv.to(device), tr, vf.on_grid(p, device),
+    return SyntheticSurvey("pstm3d-precision", p, src.to(device), rcv.to(device), tr, vf.on_grid(p, device),
                            class_sizes=sizes, info=dict(diffractors=diff, flats=flats))
 
 
-def config5_vti(seed: int = 5, n_bins: int = 200, n_classes: int = 60, class_min: int = 5000,
+def vti_largest_3d(seed: int = 5, n_bins: int = 200, n_classes: int = 60, class_min: int = 5000,
                 class_max: int = 40000, n_samples: int = 1251, n_img: int = 512, d_img: float = 12.5,
                 n_diff: int = 60, device="cpu") -> SyntheticSurvey:
     """Largest single-GPU form: VTI with a smooth random eta in [0, 0.15], 60
@@ -344,7 +344,7 @@ def config5_vti(seed: int = 5, n_bins: int = 200, n_classes: int = 60, class_min
                     eta_diff=eta_d)
     vel = vf.on_grid(p, device)  # laterally constant: broadcast to eta's layout once
     vel = vel[None, None, :].expand(n_img, n_img, n_samples).contiguous()
-    return SyntheticSurvey("cfg5-vti", p, src.to(device), rcv.to(device), tr, vel, eta=eta,
+    return SyntheticSurvey("pstm3d-vti-largest", p, src.to(device), rcv.to(device), tr, vel, eta=eta,
                            class_sizes=sizes, info=dict(diffractors=diff))
 
 
@@ -366,7 +366,7 @@ def reflectivity(gen: torch.Generator, n_il: int, n_xl: int, n_tau: int, dt: flo
     return r
 
 
-def config4_lsm(seed: int = 4, n_img: int = 256, n_tau: int = 751, d_img: float = 20.0, n_classes: int = 16,
+def lsm_3d(seed: int = 4, n_img: int = 256, n_tau: int = 751, d_img: float = 20.0, n_classes: int = 16,
                 aperture: float = 2000.0, snr: float = 10.0, p_spike: float = 0.01,
                 device="cuda") -> SyntheticSurvey:
     """Differentiation form: reflectivity n_img^2 x n_tau at d_img m and 2 ms
@@ -392,7 +392,7 @@ def config4_lsm(seed: int = 4, n_img: int = 256, n_tau: int = 751, d_img: float 
     rms = float(data.double().pow(2).mean().sqrt())
     ng = torch.Generator(device=device).manual_seed(seed + 1)
     data += (rms / snr) * torch.randn(data.shape, generator=ng, device=device, dtype=torch.float32)
-    return SyntheticSurvey("cfg4-lsm", p, src.to(device), rcv.to(device), data, vel, class_sizes=sizes,
+    return SyntheticSurvey("lsm3d", p, src.to(device), rcv.to(device), data, vel, class_sizes=sizes,
                            info=dict(reflectivity=m_true, plan=plan, snr=snr))
 
 

@@ -72,7 +72,7 @@ def test_forward_matches_oracle(name, survey_kw, param_kw):
 
 def test_forward_2d_line_and_ragged_tiles():
     # n_xl = 1 (16 x 1 tiles), n_il not a multiple of 16, n_tau not a multiple of 256
-    s = S.config1_2d(seed=1, n_cmp=37, n_off=6, n_samples=301)
+    s = S.line2d_survey(seed=1, n_cmp=37, n_off=6, n_samples=301)
     p = s.params.replace(n_tau=300)
     from seisproc_amd.processors.migration.hip_ops import PstmPlan
     plan = PstmPlan(p, s.src_xy, s.rcv_xy)
@@ -131,6 +131,34 @@ def test_forward_is_deterministic():
     b, fb = plan.forward(s.traces.to(DEV), s.vel.to(DEV))
     torch.cuda.synchronize()
     assert torch.equal(a, b) and torch.equal(fa, fb)
+
+
+def test_forward_per_band_with_progress_equals_one_call():
+    # the progress form launches one offset band at a time: same sums, bit for bit
+    s = S.tiny_survey(seed=17, n_bins=12, n_classes=4, n_samples=201, n_img=10, aperture=350.0)
+    plan = _plan(s)
+    a, fa = plan.forward(s.traces.to(DEV), s.vel.to(DEV))
+    seen = []
+    b, fb = plan.forward(s.traces.to(DEV), s.vel.to(DEV), progress_callback=lambda f, m: seen.append(f))
+    torch.cuda.synchronize()
+    assert torch.equal(a, b) and torch.equal(fa, fb)
+    assert len(seen) >= 1 and seen[-1] == 1.0 and seen == sorted(seen)
+
+
+def test_forward_band_range_is_validated():
+    import ctypes
+    from seisproc_amd.processors.kernel_backend import last_error
+    s = S.tiny_survey(seed=18, n_bins=6, n_classes=1, n_samples=101, n_img=4, aperture=200.0)
+    plan = _plan(s)
+    nb = plan.lib.pstm_plan_bands(ctypes.byref(plan.params))
+    assert nb >= 1
+    img = torch.zeros(plan.image_shape, device=DEV)
+    tr = s.traces.to(DEV)
+    vel = s.vel.to(DEV)
+    rc = plan.lib.pstm_forward_bands(ctypes.byref(plan.params), ctypes.c_void_p(plan.workspace.data_ptr()), 0,
+                                     nb + 1, ctypes.c_void_p(tr.data_ptr()), None, ctypes.c_void_p(vel.data_ptr()),
+                                     None, ctypes.c_void_p(img.data_ptr()), None, None, None)
+    assert rc < 0 and "offset bands" in last_error(plan.lib)
 
 
 def test_empty_and_unreachable_traces():
