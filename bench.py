@@ -41,12 +41,13 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per launch of the dominant kernel, from two rocprofv3 passes
-# (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on a plain 2 GiB device
-# copy (fetch x2.0, write x1.0); commands and raw counts in
+# HBM bytes per forward call of the dominant kernel (the 16 offset-band
+# dispatches of one pstm_forward), from two rocprofv3 passes (FETCH_SIZE,
+# WRITE_SIZE) scaled by the same counters on a plain 2 GiB device copy
+# (fetch x2.0, write x1.0); commands and raw counts in
 # profiles/k_tile_forward_pstm3d_precision.md.  Keyed by (workload, offset
 # classes, trace form); other runs report null.
-MEASURED_TRAFFIC = {("pstm3d-precision", 25, "bf16"): 2.4322e12}
+MEASURED_TRAFFIC = {("pstm3d-precision", 25, "bf16"): 2.728e11}
 
 
 def _parse():
