@@ -58,8 +58,9 @@ template <int NC, bool AA>
 struct PairShared {
   float2 h2[kMaxBatch][NC];     // squared source / receiver distances (m^2)
   float hm[kMaxBatch][NC];      // max of the two
-  int32_t act[kMaxBatch];       // bit w: the trace reaches some time of wave w; bit 4 + w (forward
-                                // fast path): it passes every time and column of wave w, unclamped
+  int32_t act[kMaxBatch];       // bit w: the trace reaches some time of wave w; bit 4 + w: it stays
+                                // inside the record in wave w and (forward) passes every time and
+                                // column of it; bit 8 + w (forward): passes them all but clamps
   float hcol[kWaves][NC];       // per wave and column: max hlim^2 over the wave's times
   float hcolmin[kWaves][NC];    // per wave and column: min hlim^2 over the wave's times
   float4 g[AA ? kMaxBatch : 1][AA ? NC : 1];  // antialias: scaled (s - x, r - x) components

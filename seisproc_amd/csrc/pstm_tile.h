@@ -526,7 +526,9 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           }
           bits = group_or<NC>(bits);
           if constexpr (FULLPATH) nfb = group_or<NC>(nfb);  // a wave some column does not fully pass
-          if constexpr (FULLPATH) bits |= (S.q_nc[qs + b] & ~nfb & 0xf) << 4;
+          // bits 4-7: passes every mask and stays in the record; bits 8-11: passes
+          // every mask but some time reads past the record (clamped)
+          if constexpr (FULLPATH) bits |= ((S.q_nc[qs + b] & ~nfb & 0xf) << 4) | ((~S.q_nc[qs + b] & ~nfb & 0xf) << 8);
           else if constexpr (MODE == MODE_ADJ && !AA) bits |= (S.q_nc[qs + b] & 0xf) << 4;
           if (c == NC - 1) P.act[b] = bits;
         }
@@ -541,10 +543,12 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
         // -- contributions of the traces that reach this wave's samples
         auto trace_body = [&](const int b, const int base, const float2 (&h2v)[NC], const float (&hmv)[NC],
                               auto full_tag, auto narrow_tag) {
-          // FULL: the trace's times stay inside the record in this wave (no
-          // clamp); in the forward it also passes every mask (UNMASKED)
-          constexpr bool FULL = decltype(full_tag)::value;
-          constexpr bool UNMASKED = FULL && MODE == MODE_FWD;
+          // kind 1 (FULL): the trace's times stay inside the record in this
+          // wave (no clamp); in the forward it also passes every mask.  Kind 2
+          // (forward): it passes every mask but may clamp.  Kind 0: general.
+          constexpr int KIND = decltype(full_tag)::value;
+          constexpr bool FULL = KIND == 1;
+          constexpr bool UNMASKED = KIND != 0 && MODE == MODE_FWD;
           constexpr bool NARROW = decltype(narrow_tag)::value;
           // adjoint, clamping traces: the contributions read past the record
           // end all land on samples n-2 and n-1; they are summed in a register
@@ -567,7 +571,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
             // sign is exact; any positive difference of these magnitudes,
             // hlim^2 >= (1e-6 tan(theta))^2, scales to >= 1).
             [[maybe_unused]] float mk = 1.f;
-            if constexpr (MODE == MODE_FWD && !GEN && !FULL)
+            if constexpr (MODE == MODE_FWD && !GEN && !UNMASKED)
               mk = __builtin_amdgcn_fmed3f(fmaf(hm, -0x1p64f, hl2[c] * 0x1p64f), 0.f, 1.f);
             const float xs = fmaf(h2.x, qd[c], tqs), xr = fmaf(h2.y, qd[c], tqs);
             const float rs = __builtin_amdgcn_rsqf(xs), rr = __builtin_amdgcn_rsqf(xr);
@@ -617,7 +621,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
                 a1 = Fb[i0 + 1];
                 amp = fmaf(f, a1 - a0, a0);
               }
-              if constexpr (MODE == MODE_FWD && FULL) {
+              if constexpr (MODE == MODE_FWD && UNMASKED) {
                 accb[c] = fmaf(wd, amp, accb[c]);
               } else if constexpr (MODE == MODE_FWD && !GEN) {
                 accb[c] = fmaf(wd * mk, amp, accb[c]);
@@ -769,21 +773,24 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
         uint64_t tfull = 0;
         if constexpr (FULLPATH) {
           tfull = __ballot(lane < nb && ((P.act[lane] >> (4 + wave)) & 1));
-          nfull += (int)__popcll(tfull);
-          walk(tfull, std::true_type{}, std::true_type{});
+          const uint64_t tunm = __ballot(lane < nb && ((P.act[lane] >> (8 + wave)) & 1));
+          nfull += (int)__popcll(tfull) + (int)__popcll(tunm);
+          walk(tfull, std::integral_constant<int, 1>{}, std::true_type{});
+          walk(tunm, std::integral_constant<int, 2>{}, std::true_type{});
+          tfull |= tunm;
         }
         if constexpr (MODE == MODE_ADJ && !AA) {
           // traces inside the record in this wave, then those that clamp
           tfull = __ballot(lane < nb && ((P.act[lane] >> (4 + wave)) & 1));
           if (!SKEW || narrow) {
-            walk(tfull, std::true_type{}, std::true_type{});
-            walk(todo & ~tfull, std::false_type{}, std::true_type{});
+            walk(tfull, std::integral_constant<int, 1>{}, std::true_type{});
+            walk(todo & ~tfull, std::integral_constant<int, 0>{}, std::true_type{});
           } else {
-            walk(tfull, std::true_type{}, std::false_type{});
-            walk(todo & ~tfull, std::false_type{}, std::false_type{});
+            walk(tfull, std::integral_constant<int, 1>{}, std::false_type{});
+            walk(todo & ~tfull, std::integral_constant<int, 0>{}, std::false_type{});
           }
         } else {
-          walk(todo & ~tfull, std::false_type{}, std::true_type{});
+          walk(todo & ~tfull, std::integral_constant<int, 0>{}, std::true_type{});
         }
         if constexpr (MODE != MODE_ADJ) {
           if (a.flags & PSTM_FLAG_KAHAN) {
