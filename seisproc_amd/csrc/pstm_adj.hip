@@ -31,20 +31,19 @@ int launch_adjoint(const pstm_params_t* p, const PlanLayout& L, const TileArgs& 
   // plain fixed-point windows are bank-skewed (skew_len elements)
   const int64_t fxwin = aa ? win : skew_len((int)win);
   const bool fx8 = fxwin <= window_cap_elems<8, false, 3, true>();
-  if (a.tile_x == 4 && a.tile_y == 2 && !aa && fxwin <= window_cap_elems<32, false, 2, true>()) {
-    // 3D without antialias: 8 x 4-column tiles, so that each staged trace
-    // window (flushed to the traces with global atomics) serves 32 columns
+  if (a.tile_x == 4 && a.tile_y == 2 && !aa && !gen && !vti &&
+      fxwin <= window_cap_elems<32, false, 2, true>()) {
+    // 3D, isotropic, FULL weight, no antialias: 8 x 4-column tiles, so that
+    // each staged trace window (flushed to the traces with global atomics)
+    // serves 32 columns (the general and VTI forms keep 4 x 2 tiles: with 32
+    // columns their registers spill)
     TileArgs b = a;
     b.tile_x = 8;
     b.tile_y = 4;
     b.n_tiles_x = (p->n_il + 7) / 8;
     b.n_tiles_y = (p->n_xl + 3) / 4;
-    const bool v1 = p->vel_mode == PSTM_VEL_1D;
-    rc = gen ? (vti ? launch_one<MODE_ADJ, 8, 4, PSTM_F32, false, true, true, 2, true>(b, s)
-                    : launch_one<MODE_ADJ, 8, 4, PSTM_F32, false, false, true, 2, true>(b, s))
-             : (vti ? launch_one<MODE_ADJ, 8, 4, PSTM_F32, false, true, false, 2, true>(b, s)
-                    : (v1 ? launch_one<MODE_ADJ, 8, 4, PSTM_F32, false, false, false, 2, true, true>(b, s)
-                          : launch_one<MODE_ADJ, 8, 4, PSTM_F32, false, false, false, 2, true>(b, s)));
+    rc = p->vel_mode == PSTM_VEL_1D ? launch_one<MODE_ADJ, 8, 4, PSTM_F32, false, false, false, 2, true, true>(b, s)
+                                    : launch_one<MODE_ADJ, 8, 4, PSTM_F32, false, false, false, 2, true>(b, s);
   } else if (a.tile_x == 4 && a.tile_y == 2) {
     rc = fx8 ? (gen ? PSTM_ADJ4(4, 2, true, 3, true) : PSTM_ADJ4(4, 2, false, 3, true))
              : (gen ? PSTM_ADJ4(4, 2, true, 3, false) : PSTM_ADJ4(4, 2, false, 3, false));
