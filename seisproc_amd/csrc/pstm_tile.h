@@ -571,7 +571,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
             // sign is exact; any positive difference of these magnitudes,
             // hlim^2 >= (1e-6 tan(theta))^2, scales to >= 1).
             [[maybe_unused]] float mk = 1.f;
-            if constexpr (MODE == MODE_FWD && !GEN && !UNMASKED)
+            if constexpr (MODE != MODE_GRAD && !GEN && !UNMASKED)
               mk = __builtin_amdgcn_fmed3f(fmaf(hm, -0x1p64f, hl2[c] * 0x1p64f), 0.f, 1.f);
             const float xs = fmaf(h2.x, qd[c], tqs), xr = fmaf(h2.y, qd[c], tqs);
             const float rs = __builtin_amdgcn_rsqf(xs), rr = __builtin_amdgcn_rsqf(xr);
@@ -645,12 +645,17 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               }
             } else {
               // adjoint: scatter w * m into the window (transpose of the taps)
-              float gsc = ok ? wd * m_in[c] : 0.f;
+              // branch-free masks for the common form (same FP32 predicate as
+              // the forward's); the general form selects
+              float gsc = GEN ? (ok ? wd * m_in[c] : 0.f) : wd * m_in[c] * mk;
               if constexpr (!AA && !FULL) {
-                const bool cl = u_raw > a.umax;
-                cacc += cl ? gsc : 0.f;
-                gsc = cl ? 0.f : gsc;
+                // 1 exactly when u_raw > umax (sign of the exact scaled difference)
+                const float cl = __builtin_amdgcn_fmed3f(fmaf(u_raw, 0x1p64f, -a.umax * 0x1p64f), 0.f, 1.f);
+                cacc = fmaf(gsc, cl, cacc);
+                gsc = fmaf(-gsc, cl, gsc);
               }
+              // lanes without a contribution skip the atomics: their positions
+              // may lie outside the staged window, and clamped lanes share one
               if (gsc != 0.f) {
                 if constexpr (SKEW && NARROW) {
                   // gsc < 2^30 in scaled units; each tap rounded toward zero
