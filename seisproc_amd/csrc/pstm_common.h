@@ -83,6 +83,12 @@ template <int NC, bool AA, int OCC, bool FX>
 constexpr int window_cap_elems() {
   return (FX && !AA) ? sample_cap_floats<NC, AA, OCC>() / 2 : sample_cap_elems<NC, AA, OCC>();
 }
+// The antialias forward at 3 workgroups per CU stores 16-byte window
+// elements (FP64 double integral and FP32 single integral side by side).
+template <int NC>
+constexpr int aa_fast_cap_elems() {
+  return sample_cap_floats<NC, true, 3>() / 4;
+}
 // Workgroups per CU of each kernel form: 3 for the plain tiles, 2 for the
 // antialias kernels that take any trace length (the antialias fast path
 // also has a 3-per-CU form for short traces) and for the velocity gradient.

@@ -120,24 +120,26 @@ __device__ __forceinline__ int group_or(int v) {
 // double integral of an FP32 trace is held in FP64 because the triangle taps
 // take its second difference; the same symmetric operator is its own
 // transpose (adjoint windows).
+// SA, SC: element strides of A (doubles) and C (floats).
+template <int SA = 1, int SC = 1>
 __device__ __forceinline__ void integrate_window(double* A, float* C, int len, int lane) {
   const int seg = ((len + 63) >> 6) | 1;  // odd stride: conflict-free segment reads
   const int s0 = min(lane * seg, len), s1 = min(s0 + seg, len);
   double t = 0.0;
-  for (int s = s0; s < s1; ++s) t += A[s];
+  for (int s = s0; s < s1; ++s) t += A[SA * s];
   double run = wave_scan(t, lane) - t;  // sum of the segments before this lane's
   for (int s = s0; s < s1; ++s) {
-    run += A[s];
-    A[s] = run;
-    if (C) C[s] = (float)run;
+    run += A[SA * s];
+    A[SA * s] = run;
+    if (C) C[SC * s] = (float)run;
   }
   t = 0.0;
-  for (int s = s0; s < s1; ++s) t += A[s];
+  for (int s = s0; s < s1; ++s) t += A[SA * s];
   const double incl = wave_scan(t, lane);
   run = lane63(incl) - incl;  // sum of the segments after this lane's
   for (int s = s1 - 1; s >= s0; --s) {
-    run += A[s];
-    A[s] = run;
+    run += A[SA * s];
+    A[SA * s] = run;
   }
 }
 
@@ -227,8 +229,16 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   constexpr bool PREFETCH = MODE != MODE_ADJ && !AA && !VTI && !GEN;
   static_assert(NC == 8 || (NC == 32 && MODE == MODE_ADJ), "columns per tile: 8 (32 for the adjoint)");
   static_assert(!FIXED || MODE == MODE_ADJ, "fixed-point windows are an adjoint form");
+  // ILV (antialias forward at 3 workgroups per CU): each window element is 16
+  // bytes, the FP64 double integral then the FP32 single integral, so that
+  // the three taps of a contribution need one address each
+  constexpr bool ILV = AA && MODE == MODE_FWD && OCC >= 3;
   constexpr int CAPF = sample_cap_floats<NC, AA, OCC>();         // staged-window floats
-  constexpr int CAPE = window_cap_elems<NC, AA, OCC, FX>();      // staged-window elements
+  constexpr int CAPE = ILV ? CAPF / 4 : window_cap_elems<NC, AA, OCC, FX>();  // staged-window elements
+  constexpr int SA = ILV ? 2 : 1;                                // element stride in doubles (Wa)
+  constexpr int SC = ILV ? 4 : 1;                                // element stride in floats (Wc)
+  // every staged element, FP64 A and FP32 C, lies inside samp[CAPF]
+  static_assert(!AA || (ILV ? 4 * CAPE <= CAPF : 3 * CAPE <= CAPF), "antialias window layout exceeds the LDS buffer");
   __shared__ TileShared S;
   __shared__ PairShared<NC, AA> P;
   __shared__ __align__(16) float samp[CAPF];
@@ -236,7 +246,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   // (Wa) and the FP32 single integral (Wc, forward only).
   [[maybe_unused]] float* const Wf = samp;
   [[maybe_unused]] double* const Wa = reinterpret_cast<double*>(samp);
-  [[maybe_unused]] float* const Wc = samp + 2 * CAPE;
+  [[maybe_unused]] float* const Wc = ILV ? samp + 2 : samp + 2 * CAPE;  // ILV: stride 4 floats
   [[maybe_unused]] long long* const Wl = reinterpret_cast<long long*>(samp);  // adjoint, fixed point
 
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -481,7 +491,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               for (int s = lane; s < len; s += 64) {
                 const int src = w0 + s;
                 if (src < lo || src >= hi) {
-                  if constexpr (AA) Wa[off + s] = 0.0;
+                  if constexpr (AA) Wa[SA * (off + s)] = 0.0;
                   else Wf[off + s] = 0.f;
                 }
               }
@@ -499,7 +509,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               for (int u = 0; u < 4; ++u) {
                 const int s = s0 + lane + 64 * u;
                 if (w0 + s < hi) {
-                  if constexpr (AA) Wa[off + s] = (double)(v[u] * sc);
+                  if constexpr (AA) Wa[SA * (off + s)] = (double)(v[u] * sc);
                   else Wf[off + s] = v[u] * sc;
                 }
               }
@@ -536,7 +546,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
         if constexpr (AA && MODE != MODE_ADJ) {
           // double integration (causal, then anticausal) of each window
           for (int b = wave; b < nb; b += kWaves)
-            integrate_window(Wa + S.b_off[b], Wc + S.b_off[b], S.q_len[qs + b], lane);
+            integrate_window<SA, SC>(Wa + SA * S.b_off[b], Wc + SC * S.b_off[b], S.q_len[qs + b], lane);
           __syncthreads();
         }
 
@@ -557,8 +567,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           [[maybe_unused]] float cacc = 0.f;
           // window of the trace, indexed by sample
           [[maybe_unused]] float* const Fb = Wf + base;
-          [[maybe_unused]] double* const Ab = Wa + base;
-          [[maybe_unused]] float* const Cb = Wc + base;
+          [[maybe_unused]] double* const Ab = Wa + SA * base;
+          [[maybe_unused]] float* const Cb = Wc + SC * base;
           [[maybe_unused]] unsigned long long* const Lb = reinterpret_cast<unsigned long long*>(Wl + base);
 #pragma unroll
           for (int c = 0; c < NC; ++c) {
@@ -613,8 +623,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
                 const float flm = floorf(um);
                 const int im = (int)flm, ip = (int)up;
                 const float fm = um - flm, fp = __builtin_amdgcn_fractf(up);
-                const double y2 = fma(2.0, Ab[i0], -Ab[im]) - Ab[ip];
-                const float yc = fmaf(-2.f * f, Cb[i0], fmaf(fm, Cb[im], fp * Cb[ip]));
+                const double y2 = fma(2.0, Ab[SA * i0], -Ab[SA * im]) - Ab[SA * ip];
+                const float yc = fmaf(-2.f * f, Cb[SC * i0], fmaf(fm, Cb[SC * im], fp * Cb[SC * ip]));
                 amp = ((float)y2 + yc) * iL2;
               } else {
                 a0 = Fb[i0];
@@ -1025,7 +1035,7 @@ int launch_forward_impl(const pstm_params_t* p, const PlanLayout& L, const TileA
       : (vti ? PSTM_FWD(TX, TY, false, true, GENV)      \
              : PSTM_FWD(TX, TY, false, false, GENV)))
   if (a.tile_x == 4 && a.tile_y == 2 && aa && !gen &&
-      whole_window(p, a.aa_pad) <= sample_cap_elems<8, true, 3>()) {
+      whole_window(p, a.aa_pad) <= aa_fast_cap_elems<8>()) {
     // short traces: antialias fast path at 3 workgroups per CU
     return vti ? launch_one<MODE_FWD, 4, 2, DT, true, true, false, 3>(a, s)
                : launch_one<MODE_FWD, 4, 2, DT, true, false, false, 3>(a, s);
