@@ -226,7 +226,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   constexpr bool FULLPATH = MODE == MODE_FWD && !GEN;
   // Software prefetch of the next trace's distances (registers permitting:
   // the plain forward and gradient forms).
-  constexpr bool PREFETCH = MODE != MODE_ADJ && !AA && !VTI && !GEN && (OCC <= 3 || V1);
+  constexpr bool PREFETCH = MODE != MODE_ADJ && !AA && !VTI && !GEN && (OCC <= 3 || (V1 && OCC <= 4));
   static_assert(NC == 8 || (NC == 32 && MODE == MODE_ADJ), "columns per tile: 8 (32 for the adjoint)");
   static_assert(!FIXED || MODE == MODE_ADJ, "fixed-point windows are an adjoint form");
   // ILV (antialias forward at 3 workgroups per CU): each window element is 16
@@ -1041,7 +1041,7 @@ int launch_forward_impl(const pstm_params_t* p, const PlanLayout& L, const TileA
                : launch_one<MODE_FWD, 4, 2, DT, true, false, false, 3>(a, s);
   }
   if (a.tile_x == 4 && a.tile_y == 2 && !gen && !aa && !vti)
-    return p->vel_mode == PSTM_VEL_1D ? launch_one<MODE_FWD, 4, 2, DT, false, false, false, 4, false, true>(a, s)
+    return p->vel_mode == PSTM_VEL_1D ? launch_one<MODE_FWD, 4, 2, DT, false, false, false, 5, false, true>(a, s)
                                       : launch_one<MODE_FWD, 4, 2, DT, false, false, false, 4>(a, s);
   if (a.tile_x == 4 && a.tile_y == 2) return gen ? PSTM_FWD4(4, 2, true) : PSTM_FWD4(4, 2, false);
   if (a.tile_x == 8 && a.tile_y == 1) return PSTM_FWD4(8, 1, true);
