@@ -32,7 +32,9 @@ constexpr int kQueueCap = 320;    // candidate queue entries (>= kMaxBatch - 1 +
 // (160 KB per CU, allocated in 512-byte granules).  Three workgroups of
 // 54272 bytes (159 KB) were measured to fit only two per CU; 53760 bytes
 // (157.5 KB for three) fit three.
-constexpr int lds_budget(int occ) { return occ >= 5 ? 31744 : occ >= 4 ? 39936 : occ >= 3 ? 53760 : 80896; }
+constexpr int lds_budget(int occ) {
+  return occ >= 6 ? 26112 : occ >= 5 ? 31744 : occ >= 4 ? 39936 : occ >= 3 ? 53760 : 80896;
+}
 static_assert(lds_budget(3) % 512 == 0 && lds_budget(2) % 512 == 0, "512-byte LDS granules");
 
 // Tile-kernel LDS besides the staged sample windows.
