@@ -33,7 +33,7 @@ constexpr int kQueueCap = 320;    // candidate queue entries (>= kMaxBatch - 1 +
 // 54272 bytes (159 KB) were measured to fit only two per CU; 53760 bytes
 // (157.5 KB for three) fit three.
 constexpr int lds_budget(int occ) {
-  return occ >= 6 ? 26112 : occ >= 5 ? 31744 : occ >= 4 ? 39936 : occ >= 3 ? 53760 : 80896;
+  return occ >= 8 ? 19968 : occ >= 7 ? 22528 : occ >= 6 ? 26112 : occ >= 5 ? 31744 : occ >= 4 ? 39936 : occ >= 3 ? 53760 : 80896;
 }
 static_assert(lds_budget(3) % 512 == 0 && lds_budget(2) % 512 == 0, "512-byte LDS granules");
 

@@ -1041,7 +1041,7 @@ int launch_forward_impl(const pstm_params_t* p, const PlanLayout& L, const TileA
                : launch_one<MODE_FWD, 4, 2, DT, true, false, false, 3>(a, s);
   }
   if (a.tile_x == 4 && a.tile_y == 2 && !gen && !aa && !vti)
-    return p->vel_mode == PSTM_VEL_1D ? launch_one<MODE_FWD, 4, 2, DT, false, false, false, 6, false, true>(a, s)
+    return p->vel_mode == PSTM_VEL_1D ? launch_one<MODE_FWD, 4, 2, DT, false, false, false, 7, false, true>(a, s)
                                       : launch_one<MODE_FWD, 4, 2, DT, false, false, false, 4>(a, s);
   if (a.tile_x == 4 && a.tile_y == 2) return gen ? PSTM_FWD4(4, 2, true) : PSTM_FWD4(4, 2, false);
   if (a.tile_x == 8 && a.tile_y == 1) return PSTM_FWD4(8, 1, true);
