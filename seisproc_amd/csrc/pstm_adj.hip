@@ -31,7 +31,15 @@ int launch_adjoint(const pstm_params_t* p, const PlanLayout& L, const TileArgs& 
   // plain fixed-point windows are bank-skewed (skew_len elements)
   const int64_t fxwin = aa ? win : skew_len((int)win);
   const bool fx8 = fxwin <= window_cap_elems<8, false, 3, true>();
-  if (a.tile_x == 4 && a.tile_y == 2 && !aa && !gen && !vti &&
+  // Plain 3D forms: occupancy first (measured on the 4-class LSM slice: 4 x 2
+  // tiles at 7 workgroups per CU 831 Gcontrib/s, at 4 752, 8 x 4 tiles at 2
+  // 717); longer traces fall back to the wider tiles' larger windows.
+  if (a.tile_x == 4 && a.tile_y == 2 && !aa && !gen && !vti && p->vel_mode == PSTM_VEL_1D &&
+      fxwin <= window_cap_elems<8, false, 7, true>()) {
+    rc = launch_one<MODE_ADJ, 4, 2, PSTM_F32, false, false, false, 7, true, true>(a, s);
+  } else if (a.tile_x == 4 && a.tile_y == 2 && !aa && !gen && !vti && fxwin <= window_cap_elems<8, false, 4, true>()) {
+    rc = launch_one<MODE_ADJ, 4, 2, PSTM_F32, false, false, false, 4, true>(a, s);
+  } else if (a.tile_x == 4 && a.tile_y == 2 && !aa && !gen && !vti &&
       fxwin <= window_cap_elems<32, false, 2, true>()) {
     // 3D, isotropic, FULL weight, no antialias: 8 x 4-column tiles, so that
     // each staged trace window (flushed to the traces with global atomics)
