@@ -20,31 +20,35 @@ constexpr int kTileTau = kThreads;
 // rigorous traveltime bounds (absorbs FP32 rounding of the bounds).
 constexpr int kWindowMargin = 2;
 
-constexpr int kMaxBatch = 64;     // traces per sub-batch
+// Traces per sub-batch: at high occupancy the small LDS share holds the
+// windows of only ~10 traces, so the per-batch arrays shrink with it.
+constexpr int batch_of(int occ) { return occ >= 6 ? 16 : occ >= 4 ? 32 : 64; }
 // The plan orders traces by (offset band, midpoint cell) and the tile
 // operators run one launch per band, so the grid reads traces of one band at
 // a time (a working set that stays in the caches; measured 5-7 % faster than
 // one launch over all offsets on the 1M-trace, 25-class workload).
 constexpr int kOffsetBands = 16;
-constexpr int kQueueCap = 320;    // candidate queue entries (>= kMaxBatch - 1 + kThreads)
+// Candidate queue entries: a batch's remainder (< MB) plus one chunk of kThreads.
+constexpr int queue_cap(int mb) { return ((mb - 1 + kThreads) + 15) & ~15; }
 
-// LDS budget of one tile workgroup for OCC (2 or 3) workgroups per CU
+// LDS budget of one tile workgroup for OCC (2 to 8) workgroups per CU
 // (160 KB per CU, allocated in 512-byte granules).  Three workgroups of
 // 54272 bytes (159 KB) were measured to fit only two per CU; 53760 bytes
-// (157.5 KB for three) fit three.
+// (157.5 KB for three) fit three; the others keep the same 1.5 KB of slack.
 constexpr int lds_budget(int occ) {
   return occ >= 8 ? 19968 : occ >= 7 ? 22528 : occ >= 6 ? 26112 : occ >= 5 ? 31744 : occ >= 4 ? 39936 : occ >= 3 ? 53760 : 80896;
 }
 static_assert(lds_budget(3) % 512 == 0 && lds_budget(2) % 512 == 0, "512-byte LDS granules");
 
-// Tile-kernel LDS besides the staged sample windows.
+// Tile-kernel LDS besides the staged sample windows (MB traces per batch).
+template <int MB>
 struct TileShared {
-  int32_t q_j[kQueueCap];       // candidate queue: sorted trace index
-  int32_t q_w0[kQueueCap];      // first staged sample
-  int32_t q_len[kQueueCap];     // staged window length
-  uint8_t q_nc[kQueueCap];      // bit w: no time of wave w reads past either end of the record
-  int32_t b_base[kMaxBatch];    // window-element index of sample 0 of the trace (off - w0)
-  int32_t b_off[kMaxBatch];     // window-element offset of the window
+  int32_t q_j[queue_cap(MB)];   // candidate queue: sorted trace index
+  int32_t q_w0[queue_cap(MB)];  // first staged sample
+  int32_t q_len[queue_cap(MB)]; // staged window length
+  uint8_t q_nc[queue_cap(MB)];  // bit w: no time of wave w reads past either end of the record
+  int32_t b_base[MB];           // window-element index of sample 0 of the trace (off - w0)
+  int32_t b_off[MB];            // window-element offset of the window
   int32_t scan[kWaves];
   int32_t queue_n;
   int32_t nb;
@@ -56,16 +60,16 @@ struct TileShared {
 };
 
 // Per sub-batch (trace, column) quantities of a tile of NC columns.
-template <int NC, bool AA>
+template <int NC, bool AA, int MB>
 struct PairShared {
-  float2 h2[kMaxBatch][NC];     // squared source / receiver distances (m^2)
-  float hm[kMaxBatch][NC];      // max of the two
-  int32_t act[kMaxBatch];       // bit w: the trace reaches some time of wave w; bit 4 + w: it stays
+  float2 h2[MB][NC];            // squared source / receiver distances (m^2)
+  float hm[MB][NC];             // max of the two
+  int32_t act[MB];              // bit w: the trace reaches some time of wave w; bit 4 + w: it stays
                                 // inside the record in wave w and (forward) passes every time and
                                 // column of it; bit 8 + w (forward): passes them all but clamps
   float hcol[kWaves][NC];       // per wave and column: max hlim^2 over the wave's times
   float hcolmin[kWaves][NC];    // per wave and column: min hlim^2 over the wave's times
-  float4 g[AA ? kMaxBatch : 1][AA ? NC : 1];  // antialias: scaled (s - x, r - x) components
+  float4 g[AA ? MB : 1][AA ? NC : 1];  // antialias: scaled (s - x, r - x) components
 };
 
 // Staged-window capacity in floats and in window elements.  A window
@@ -73,7 +77,8 @@ struct PairShared {
 // integral plus the FP32 single integral (three floats).
 template <int NC, bool AA, int OCC>
 constexpr int sample_cap_floats() {
-  return ((lds_budget(OCC) - (int)sizeof(TileShared) - (int)sizeof(PairShared<NC, AA>)) / 4) & ~63;
+  return ((lds_budget(OCC) - (int)sizeof(TileShared<batch_of(OCC)>) -
+           (int)sizeof(PairShared<NC, AA, batch_of(OCC)>)) / 4) & ~63;
 }
 template <int NC, bool AA, int OCC>
 constexpr int sample_cap_elems() {

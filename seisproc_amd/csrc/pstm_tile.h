@@ -239,8 +239,9 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   constexpr int SC = ILV ? 4 : 1;                                // element stride in floats (Wc)
   // every staged element, FP64 A and FP32 C, lies inside samp[CAPF]
   static_assert(!AA || (ILV ? 4 * CAPE <= CAPF : 3 * CAPE <= CAPF), "antialias window layout exceeds the LDS buffer");
-  __shared__ TileShared S;
-  __shared__ PairShared<NC, AA> P;
+  constexpr int MB = batch_of(OCC);  // traces per sub-batch
+  __shared__ TileShared<MB> S;
+  __shared__ PairShared<NC, AA, MB> P;
   __shared__ __align__(16) float samp[CAPF];
   // Staged windows: FP32 samples, or with antialias the FP64 double integral
   // (Wa) and the FP32 single integral (Wc, forward only).
@@ -449,10 +450,10 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
       const int qn = S.queue_n;
       while (true) {
         const int rem = qn - qs;
-        if (rem <= 0 || (!final_flush && rem < kMaxBatch)) break;
+        if (rem <= 0 || (!final_flush && rem < MB)) break;
         if (wave == 0) {
           const int i = qs + lane;
-          const bool in = lane < kMaxBatch && i < qn;
+          const bool in = lane < MB && i < qn;
           const int cost = in ? (SKEW ? skew_len(S.q_len[i]) : S.q_len[i]) : 0;  // window elements
           const int incl = wave_scan(cost, lane);
           const bool fit = in && incl <= CAPE;
@@ -860,7 +861,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
         }
         qs += nb;
       }
-      // move the unprocessed tail (< kMaxBatch entries) to the front
+      // move the unprocessed tail (< MB entries) to the front
       const int rem = qn - qs;
       int j = 0, w0 = 0, len = 0, nc = 0;
       if (tid < rem) {
@@ -954,7 +955,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
         __syncthreads();
         if (tid == 0) S.queue_n = qn + tot;
         __syncthreads();
-        if (qn + tot >= kMaxBatch) process_queue(false);
+        if (qn + tot >= MB) process_queue(false);
       }
     }
     process_queue(true);
@@ -1040,9 +1041,16 @@ int launch_forward_impl(const pstm_params_t* p, const PlanLayout& L, const TileA
     return vti ? launch_one<MODE_FWD, 4, 2, DT, true, true, false, 3>(a, s)
                : launch_one<MODE_FWD, 4, 2, DT, true, false, false, 3>(a, s);
   }
-  if (a.tile_x == 4 && a.tile_y == 2 && !gen && !aa && !vti)
-    return p->vel_mode == PSTM_VEL_1D ? launch_one<MODE_FWD, 4, 2, DT, false, false, false, 7, false, true>(a, s)
-                                      : launch_one<MODE_FWD, 4, 2, DT, false, false, false, 4>(a, s);
+  // Plain forward at high occupancy (measured on the 1M-trace workload: 1D
+  // velocity at 7 workgroups per CU 1930 Gcontrib/s, 6: 1885, 5: 1849, 4 with
+  // prefetch 1814, 3 with prefetch 1624, 8: 1870).  A sub-batch must hold a
+  // whole trace's window, so the smaller LDS shares serve traces that fit.
+  const int64_t win = whole_window(p, 0);
+  if (a.tile_x == 4 && a.tile_y == 2 && !gen && !aa && !vti) {
+    if (p->vel_mode == PSTM_VEL_1D && win <= sample_cap_elems<8, false, 7>())
+      return launch_one<MODE_FWD, 4, 2, DT, false, false, false, 7, false, true>(a, s);
+    if (win <= sample_cap_elems<8, false, 4>()) return launch_one<MODE_FWD, 4, 2, DT, false, false, false, 4>(a, s);
+  }
   if (a.tile_x == 4 && a.tile_y == 2) return gen ? PSTM_FWD4(4, 2, true) : PSTM_FWD4(4, 2, false);
   if (a.tile_x == 8 && a.tile_y == 1) return PSTM_FWD4(8, 1, true);
   if (a.tile_x == 1 && a.tile_y == 8) return PSTM_FWD4(1, 8, true);
