@@ -92,9 +92,9 @@ constexpr int window_cap_elems() {
 }
 // The antialias forward at 3 workgroups per CU stores 16-byte window
 // elements (FP64 double integral and FP32 single integral side by side).
-template <int NC>
+template <int NC, int OCC = 3>
 constexpr int aa_fast_cap_elems() {
-  return sample_cap_floats<NC, true, 3>() / 4;
+  return sample_cap_floats<NC, true, OCC>() / 4;
 }
 // Workgroups per CU of each kernel form: 3 for the plain tiles, 2 for the
 // antialias kernels that take any trace length (the antialias fast path

@@ -1036,8 +1036,14 @@ int launch_forward_impl(const pstm_params_t* p, const PlanLayout& L, const TileA
       : (vti ? PSTM_FWD(TX, TY, false, true, GENV)      \
              : PSTM_FWD(TX, TY, false, false, GENV)))
   if (a.tile_x == 4 && a.tile_y == 2 && aa && !gen &&
+      whole_window(p, a.aa_pad) <= aa_fast_cap_elems<8, 4>()) {
+    // short traces: antialias fast path at 4 workgroups per CU
+    return vti ? launch_one<MODE_FWD, 4, 2, DT, true, true, false, 4>(a, s)
+               : launch_one<MODE_FWD, 4, 2, DT, true, false, false, 4>(a, s);
+  }
+  if (a.tile_x == 4 && a.tile_y == 2 && aa && !gen &&
       whole_window(p, a.aa_pad) <= aa_fast_cap_elems<8>()) {
-    // short traces: antialias fast path at 3 workgroups per CU
+    // antialias fast path at 3 workgroups per CU
     return vti ? launch_one<MODE_FWD, 4, 2, DT, true, true, false, 3>(a, s)
                : launch_one<MODE_FWD, 4, 2, DT, true, false, false, 3>(a, s);
   }
