@@ -96,9 +96,11 @@ template <int NC, int OCC = 3>
 constexpr int aa_fast_cap_elems() {
   return sample_cap_floats<NC, true, OCC>() / 4;
 }
-// Workgroups per CU of each kernel form: 3 for the plain tiles, 2 for the
-// antialias kernels that take any trace length (the antialias fast path
-// also has a 3-per-CU form for short traces) and for the velocity gradient.
+// Workgroups per CU of the kernel forms that take any admissible trace
+// length: 3 for the plain tiles and the velocity gradient, 2 for antialias.
+// Traces whose whole window fits a smaller LDS share run at higher
+// occupancy: the plain forward at 4 (7 with a velocity per output time),
+// the plain adjoint at 4 (7), the antialias forward at 4 (else 3).
 constexpr int default_occ(bool aa) { return aa ? 2 : 3; }
 // Capacity every call can rely on (8-column tiles).
 constexpr int min_cap_elems(bool aa) {
