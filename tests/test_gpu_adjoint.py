@@ -42,6 +42,27 @@ def test_adjoint_dot_product(flags):
     assert rel <= 1e-5
 
 
+@pytest.mark.parametrize("n_samples", [1500, 3000, 5000])
+def test_adjoint_dot_product_long_traces(n_samples):
+    # plain adjoint tiers by trace length: 4x2 tiles at 7 per CU (1D velocity,
+    # windows up to ~2000 samples), at 4 (~4000), 8x4 tiles at 2 (~6200)
+    from seisproc_amd.processors.migration.hip_ops import PstmPlan
+    s = S.tiny_survey(seed=23, n_bins=8, n_classes=2, n_samples=n_samples, n_img=6, aperture=300.0)
+    p = s.params.replace(n_tau=min(n_samples, 900))
+    vel = s.vel[:p.n_tau].contiguous().to(DEV)
+    plan = PstmPlan(p, s.src_xy, s.rcv_xy)
+    g = torch.Generator(device=DEV).manual_seed(6)
+    d = torch.randn((p.n_traces, p.n_samples), generator=g, device=DEV)
+    m = torch.randn(plan.image_shape, generator=g, device=DEV)
+    Ld, _ = plan.forward(d, vel, with_fold=False)
+    Ltm = plan.adjoint(m, vel)
+    lhs = float((Ld.double() * m.double()).sum())
+    rhs = float((d.double() * Ltm.double()).sum())
+    rel = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    print(f"[dot n_samples={n_samples}] rel={rel:.2e}")
+    assert rel <= 1e-5
+
+
 @pytest.mark.parametrize("flags", [0, FLAG_AA])
 def test_adjoint_matches_oracle(flags):
     s, plan, p, vel = _setup(flags)
