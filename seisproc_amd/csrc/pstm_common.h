@@ -96,6 +96,15 @@ template <int NC, int OCC = 3>
 constexpr int aa_fast_cap_elems() {
   return sample_cap_floats<NC, true, OCC>() / 4;
 }
+// The plain forward at 4 or more workgroups per CU stores 8-byte window
+// elements: the sample and its forward difference.
+template <int OCC>
+constexpr int pair_cap_elems() {
+  return sample_cap_floats<8, false, OCC>() / 2;
+}
+// Word 3 of a raw buffer resource (gfx9: 32-bit data format, no swizzle,
+// stride 0): loads at byte offsets at or past num_records return 0.
+constexpr int kRawBufferWord3 = 0x00020000;
 // Workgroups per CU of the kernel forms that take any admissible trace
 // length: 3 for the plain tiles and the velocity gradient, 2 for antialias.
 // Traces whose whole window fits a smaller LDS share run at higher

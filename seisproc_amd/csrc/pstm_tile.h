@@ -169,6 +169,20 @@ __device__ __forceinline__ float load_sample(const void* tr, int64_t idx) {
   }
 }
 
+// Sample at byte offset `boff` of a raw buffer over one trace (0 at and past
+// its end).
+template <int DT>
+__device__ __forceinline__ float buffer_sample(__amdgpu_buffer_rsrc_t r, int boff) {
+  if constexpr (DT == PSTM_F32) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, boff, 0, 0));
+  } else if constexpr (DT == PSTM_BF16) {
+    return __uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, boff, 0, 0) << 16);
+  } else {
+    const uint16_t b = __builtin_amdgcn_raw_buffer_load_b16(r, boff, 0, 0);
+    return (float)__builtin_bit_cast(_Float16, b);
+  }
+}
+
 // Neumaier two-term update: (s, c) += y.
 __device__ __forceinline__ void neumaier(float& s, float& c, float y) {
   float t = s + y;
@@ -233,8 +247,13 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   // bytes, the FP64 double integral then the FP32 single integral, so that
   // the three taps of a contribution need one address each
   constexpr bool ILV = AA && MODE == MODE_FWD && OCC >= 3;
+  // PAIR (plain forward at 4 or more workgroups per CU): each window element
+  // is a sample and its forward difference (a[i], a[i+1] - a[i]) in FP32, so
+  // a contribution reads one 8-byte element and interpolates with one FMA
+  // (the same FP32 difference the single-sample layout forms per read)
+  constexpr bool PAIR = MODE == MODE_FWD && !AA && !GEN && !VTI && OCC >= 4;
   constexpr int CAPF = sample_cap_floats<NC, AA, OCC>();         // staged-window floats
-  constexpr int CAPE = ILV ? CAPF / 4 : window_cap_elems<NC, AA, OCC, FX>();  // staged-window elements
+  constexpr int CAPE = ILV ? CAPF / 4 : PAIR ? CAPF / 2 : window_cap_elems<NC, AA, OCC, FX>();  // staged-window elements
   constexpr int SA = ILV ? 2 : 1;                                // element stride in doubles (Wa)
   constexpr int SC = ILV ? 4 : 1;                                // element stride in floats (Wc)
   // every staged element, FP64 A and FP32 C, lies inside samp[CAPF]
@@ -246,6 +265,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   // Staged windows: FP32 samples, or with antialias the FP64 double integral
   // (Wa) and the FP32 single integral (Wc, forward only).
   [[maybe_unused]] float* const Wf = samp;
+  [[maybe_unused]] float2* const Wp = reinterpret_cast<float2*>(samp);  // PAIR elements
   [[maybe_unused]] double* const Wa = reinterpret_cast<double*>(samp);
   [[maybe_unused]] float* const Wc = ILV ? samp + 2 : samp + 2 * CAPE;  // ILV: stride 4 floats
   [[maybe_unused]] long long* const Wl = reinterpret_cast<long long*>(samp);  // adjoint, fixed point
@@ -487,6 +507,35 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
             const int64_t row = (int64_t)id * a.n_samples;
             // in-record part [lo, hi) of the window; antialias windows also
             // hold zero padding outside the record
+            if constexpr (PAIR) {
+              // the window lies inside the record; the difference at the
+              // record's last sample reads one sample past it, which the
+              // buffer range check returns as 0 (that element is never
+              // interpolated: reads stop at sample n - 2).  Products and
+              // difference are rounded as the single-sample layout rounds them.
+              constexpr int ES = DT == PSTM_F32 ? 4 : 2;
+              const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                  const_cast<char*>(static_cast<const char*>(a.traces)) + row * ES, 0, a.n_samples * ES,
+                  kRawBufferWord3);
+              for (int s0 = 0; s0 < len; s0 += 4 * 64) {
+                float v[4], vn[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                  const int boff = (w0 + s0 + lane + 64 * u) * ES;
+                  v[u] = buffer_sample<DT>(rs, boff);
+                  vn[u] = buffer_sample<DT>(rs, boff + ES);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                  const int s = s0 + lane + 64 * u;
+                  if (s < len) {
+                    const float x = __fmul_rn(v[u], sc);
+                    Wp[off + s] = make_float2(x, __fsub_rn(__fmul_rn(vn[u], sc), x));
+                  }
+                }
+              }
+              continue;
+            }
             const int lo = max(w0, 0), hi = min(w0 + len, a.n_samples);
             if (lo > w0 || hi < w0 + len) {
               for (int s = lane; s < len; s += 64) {
@@ -568,6 +617,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           [[maybe_unused]] float cacc = 0.f;
           // window of the trace, indexed by sample
           [[maybe_unused]] float* const Fb = Wf + base;
+          [[maybe_unused]] const float2* const Pb = Wp + base;
           [[maybe_unused]] double* const Ab = Wa + SA * base;
           [[maybe_unused]] float* const Cb = Wc + SC * base;
           [[maybe_unused]] unsigned long long* const Lb = reinterpret_cast<unsigned long long*>(Wl + base);
@@ -627,6 +677,9 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
                 const double y2 = fma(2.0, Ab[SA * i0], -Ab[SA * im]) - Ab[SA * ip];
                 const float yc = fmaf(-2.f * f, Cb[SC * i0], fmaf(fm, Cb[SC * im], fp * Cb[SC * ip]));
                 amp = ((float)y2 + yc) * iL2;
+              } else if constexpr (PAIR) {
+                const float2 e = Pb[i0];
+                amp = fmaf(f, e.y, e.x);
               } else {
                 a0 = Fb[i0];
                 a1 = Fb[i0 + 1];
@@ -1053,9 +1106,9 @@ int launch_forward_impl(const pstm_params_t* p, const PlanLayout& L, const TileA
   // whole trace's window, so the smaller LDS shares serve traces that fit.
   const int64_t win = whole_window(p, 0);
   if (a.tile_x == 4 && a.tile_y == 2 && !gen && !aa && !vti) {
-    if (p->vel_mode == PSTM_VEL_1D && win <= sample_cap_elems<8, false, 7>())
+    if (p->vel_mode == PSTM_VEL_1D && win <= pair_cap_elems<7>())
       return launch_one<MODE_FWD, 4, 2, DT, false, false, false, 7, false, true>(a, s);
-    if (win <= sample_cap_elems<8, false, 4>()) return launch_one<MODE_FWD, 4, 2, DT, false, false, false, 4>(a, s);
+    if (win <= pair_cap_elems<4>()) return launch_one<MODE_FWD, 4, 2, DT, false, false, false, 4>(a, s);
   }
   if (a.tile_x == 4 && a.tile_y == 2) return gen ? PSTM_FWD4(4, 2, true) : PSTM_FWD4(4, 2, false);
   if (a.tile_x == 8 && a.tile_y == 1) return PSTM_FWD4(8, 1, true);
