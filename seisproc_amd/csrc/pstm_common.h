@@ -28,6 +28,11 @@ constexpr int batch_of(int occ) { return occ >= 6 ? 16 : occ >= 4 ? 32 : 64; }
 // a time (a working set that stays in the caches; measured 5-7 % faster than
 // one launch over all offsets on the 1M-trace, 25-class workload).
 constexpr int kOffsetBands = 16;
+// Traces summed in FP32 per wave before the partial sums join the running
+// (compensated) sums: a flush costs about one trace's arithmetic per column,
+// so flushing after every small sub-batch of the high-occupancy forms cost 4 %
+// of the forward time.
+constexpr int kFlushTraces = 32;
 // Candidate queue entries: a batch's remainder (< MB) plus one chunk of kThreads.
 constexpr int queue_cap(int mb) { return ((mb - 1 + kThreads) + 15) & ~15; }
 

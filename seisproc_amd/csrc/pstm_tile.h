@@ -183,10 +183,15 @@ __device__ __forceinline__ float buffer_sample(__amdgpu_buffer_rsrc_t r, int bof
   }
 }
 
-// Neumaier two-term update: (s, c) += y.
-__device__ __forceinline__ void neumaier(float& s, float& c, float y) {
-  float t = s + y;
-  c += (fabsf(s) >= fabsf(y)) ? ((s - t) + y) : ((y - t) + s);
+// Compensated two-term update: (s, c) += y.
+// Knuth's branch-free TwoSum recovers the rounding error of s + y exactly
+// with six additions (no magnitude test); contraction stays off so that no
+// addition is fused with the product forming y.
+__device__ __forceinline__ void two_sum_add(float& s, float& c, float y) {
+#pragma clang fp contract(off)
+  const float t = s + y;
+  const float z = t - s;
+  c += (s - (t - z)) + (y - z);
   s = t;
 }
 
@@ -465,6 +470,25 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
 
     // Process the queued traces in LDS-sized sub-batches.  `final_flush`:
     // also process a last partial batch.
+    // Partial sums (accb) of at least kFlushTraces traces per wave, summed in
+    // FP32, are added to the running sums (acc) between sub-batches, with the
+    // compensated update under PSTM_FLAG_KAHAN.
+    int nsum = 0;
+    auto flush = [&]() {
+      if (a.flags & PSTM_FLAG_KAHAN) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          two_sum_add(acc[c], cmp[c], accb[c] * cw[c]);
+          accb[c] = 0.f;
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          acc[c] = fmaf(accb[c], cw[c], acc[c]);
+          accb[c] = 0.f;
+        }
+      }
+    };
     auto process_queue = [&](bool final_flush) {
       int qs = 0;
       const int qn = S.queue_n;
@@ -508,6 +532,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
             // in-record part [lo, hi) of the window; antialias windows also
             // hold zero padding outside the record
             if constexpr (PAIR) {
+#pragma clang fp contract(off)
               // the window lies inside the record; the difference at the
               // record's last sample reads one sample past it, which the
               // buffer range check returns as 0 (that element is never
@@ -529,8 +554,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
                 for (int u = 0; u < 4; ++u) {
                   const int s = s0 + lane + 64 * u;
                   if (s < len) {
-                    const float x = __fmul_rn(v[u], sc);
-                    Wp[off + s] = make_float2(x, __fsub_rn(__fmul_rn(vn[u], sc), x));
+                    const float x = v[u] * sc;
+                    Wp[off + s] = make_float2(x, vn[u] * sc - x);
                   }
                 }
               }
@@ -862,18 +887,10 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           walk(todo & ~tfull, std::integral_constant<int, 0>{}, std::true_type{});
         }
         if constexpr (MODE != MODE_ADJ) {
-          if (a.flags & PSTM_FLAG_KAHAN) {
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-              neumaier(acc[c], cmp[c], accb[c] * cw[c]);
-              accb[c] = 0.f;
-            }
-          } else {
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-              acc[c] = fmaf(accb[c], cw[c], acc[c]);
-              accb[c] = 0.f;
-            }
+          nsum += __popcll(todo);
+          if (nsum >= kFlushTraces) {
+            flush();
+            nsum = 0;
           }
         }
         __syncthreads();
@@ -1012,6 +1029,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
       }
     }
     process_queue(true);
+    if constexpr (MODE != MODE_ADJ) flush();
   }
 
   // ---- write-out
@@ -1024,7 +1042,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
     if constexpr (MODE == MODE_FWD) {
       if (a.comp) {
         float s = a.image[idx], cc = a.comp[idx];
-        neumaier(s, cc, acc[c]);
+        two_sum_add(s, cc, acc[c]);
         a.image[idx] = s;
         a.comp[idx] = cc + cmp[c];
       } else {
