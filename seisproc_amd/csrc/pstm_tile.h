@@ -529,24 +529,23 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
             const int id = a.ids[S.q_j[e]];
             const float sc = a.scale ? a.scale[id] : 1.f;
             const int64_t row = (int64_t)id * a.n_samples;
-            // in-record part [lo, hi) of the window; antialias windows also
-            // hold zero padding outside the record
             if constexpr (PAIR) {
 #pragma clang fp contract(off)
-              // the window lies inside the record; the difference at the
-              // record's last sample reads one sample past it, which the
-              // buffer range check returns as 0 (that element is never
-              // interpolated: reads stop at sample n - 2).  Products and
-              // difference are rounded as the single-sample layout rounds them.
+              // A raw buffer over the window and the sample after it (the
+              // window lies inside the record): the lanes past the window end
+              // fetch nothing (the range check returns 0), and at the record's
+              // last sample the difference reads 0 (never interpolated: reads
+              // stop at sample n - 2).  Products and difference are rounded as
+              // the single-sample layout rounds them.
               constexpr int ES = DT == PSTM_F32 ? 4 : 2;
               const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-                  const_cast<char*>(static_cast<const char*>(a.traces)) + row * ES, 0, a.n_samples * ES,
-                  kRawBufferWord3);
+                  const_cast<char*>(static_cast<const char*>(a.traces)) + (row + w0) * ES, 0,
+                  min(len + 1, a.n_samples - w0) * ES, kRawBufferWord3);
               for (int s0 = 0; s0 < len; s0 += 4 * 64) {
                 float v[4], vn[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                  const int boff = (w0 + s0 + lane + 64 * u) * ES;
+                  const int boff = (s0 + lane + 64 * u) * ES;
                   v[u] = buffer_sample<DT>(rs, boff);
                   vn[u] = buffer_sample<DT>(rs, boff + ES);
                 }
@@ -561,6 +560,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               }
               continue;
             }
+            // in-record part [lo, hi) of the window; antialias windows also
+            // hold zero padding outside the record
             const int lo = max(w0, 0), hi = min(w0 + len, a.n_samples);
             if (lo > w0 || hi < w0 + len) {
               for (int s = lane; s < len; s += 64) {
