@@ -47,7 +47,7 @@ HBM_PEAK_GBS = 8000.0
 # (fetch x2.0, write x1.0); commands and raw counts in
 # profiles/k_tile_forward_pstm3d_precision.md.  Keyed by (workload, offset
 # classes, trace form); other runs report null.
-MEASURED_TRAFFIC = {("pstm3d-precision", 25, "bf16"): 2.731e12}
+MEASURED_TRAFFIC = {("pstm3d-precision", 25, "bf16"): 3.447e12}
 
 
 def _parse():
