@@ -468,13 +468,11 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
     const int pad = AA ? a.aa_pad : 0;
     const int wlo = -pad - kWindowMargin, whi = a.n_samples - 1 + pad + kWindowMargin;
 
-    // Process the queued traces in LDS-sized sub-batches.  `final_flush`:
-    // also process a last partial batch.
     // Partial sums (accb) of at least kFlushTraces traces per wave, summed in
     // FP32, are added to the running sums (acc) between sub-batches, with the
     // compensated update under PSTM_FLAG_KAHAN.
     int nsum = 0;
-    auto flush = [&]() {
+    auto join_partials = [&]() {
       if (a.flags & PSTM_FLAG_KAHAN) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
@@ -489,6 +487,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
         }
       }
     };
+    // Process the queued traces in LDS-sized sub-batches.  `final_flush`:
+    // also process a last partial batch.
     auto process_queue = [&](bool final_flush) {
       int qs = 0;
       const int qn = S.queue_n;
@@ -890,7 +890,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
         if constexpr (MODE != MODE_ADJ) {
           nsum += __popcll(todo);
           if (nsum >= kFlushTraces) {
-            flush();
+            join_partials();
             nsum = 0;
           }
         }
@@ -1030,7 +1030,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
       }
     }
     process_queue(true);
-    if constexpr (MODE != MODE_ADJ) flush();
+    if constexpr (MODE != MODE_ADJ) join_partials();
   }
 
   // ---- write-out
