@@ -1121,12 +1121,15 @@ int launch_forward_impl(const pstm_params_t* p, const PlanLayout& L, const TileA
   }
   // Plain forward at high occupancy (measured on the 1M-trace workload: 1D
   // velocity at 7 workgroups per CU 1930 Gcontrib/s, 6: 1885, 5: 1849, 4 with
-  // prefetch 1814, 3 with prefetch 1624, 8: 1870).  A sub-batch must hold a
-  // whole trace's window, so the smaller LDS shares serve traces that fit.
+  // prefetch 1814, 3 with prefetch 1624, 8: 1870; 3D velocity on 8 of its 25
+  // classes, sample-and-difference windows: 6 per CU 1828, 5: 1823, 4: 1707).
+  // A sub-batch must hold a whole trace's window, so the smaller LDS shares
+  // serve traces that fit.
   const int64_t win = whole_window(p, 0);
   if (a.tile_x == 4 && a.tile_y == 2 && !gen && !aa && !vti) {
     if (p->vel_mode == PSTM_VEL_1D && win <= pair_cap_elems<7>())
       return launch_one<MODE_FWD, 4, 2, DT, false, false, false, 7, false, true>(a, s);
+    if (win <= pair_cap_elems<6>()) return launch_one<MODE_FWD, 4, 2, DT, false, false, false, 6>(a, s);
     if (win <= pair_cap_elems<4>()) return launch_one<MODE_FWD, 4, 2, DT, false, false, false, 4>(a, s);
   }
   if (a.tile_x == 4 && a.tile_y == 2) return gen ? PSTM_FWD4(4, 2, true) : PSTM_FWD4(4, 2, false);

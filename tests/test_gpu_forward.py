@@ -176,14 +176,15 @@ def test_empty_and_unreachable_traces():
     assert pl.kept() == 0 and float(fold.sum()) == 0
 
 
-@pytest.mark.parametrize("n_samples,flags", [(10500, 0), (6000, 0), (4027, 0), (4026, 0), (3000, 0), (2075, 0),
-                                             (2074, 0), (5000, FLAG_AA), (1900, FLAG_AA)])
+@pytest.mark.parametrize("n_samples,flags", [(10500, 0), (6000, 0), (4027, 0), (4026, 0), (3000, 0), (2523, 0),
+                                             (2522, 0), (2075, 0), (2074, 0), (5000, FLAG_AA), (1900, FLAG_AA)])
 def test_long_traces_near_lds_limit(n_samples, flags):
     # windows of whole traces close to the LDS sample capacity (10554 plain,
     # 5056 with antialias at aa_max 16); the shorter lengths take the
     # higher-occupancy forms' fallbacks and their capacity edges (plain,
-    # sample-and-difference windows: 7 per CU up to 2074 samples, 4 up to
-    # 4026, else 3; antialias: 4 per CU up to ~1720, then 3)
+    # sample-and-difference windows: 7 per CU up to 2074 samples with a
+    # velocity per output time, 6 up to 2522, 4 up to 4026, else 3;
+    # antialias: 4 per CU up to ~1720, then 3)
     s = S.tiny_survey(seed=15, n_bins=6, n_classes=1, n_samples=n_samples, n_img=4, aperture=300.0)
     p = s.params.replace(n_tau=700, flags=flags)
     vel = s.vel[:700].contiguous()
