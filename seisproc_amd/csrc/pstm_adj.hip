@@ -33,10 +33,13 @@ int launch_adjoint(const pstm_params_t* p, const PlanLayout& L, const TileArgs& 
   const bool fx8 = fxwin <= window_cap_elems<8, false, 3, true>();
   // Plain 3D forms: occupancy first (measured on the 4-class LSM slice: 4 x 2
   // tiles at 7 workgroups per CU 831 Gcontrib/s, at 4 752, 8 x 4 tiles at 2
-  // 717); longer traces fall back to the wider tiles' larger windows.
+  // 717; with a laterally varying velocity 4 x 2 tiles at 6 per CU 803, at 5
+  // 780, at 4 735); longer traces fall back to the wider tiles' larger windows.
   if (a.tile_x == 4 && a.tile_y == 2 && !aa && !gen && !vti && p->vel_mode == PSTM_VEL_1D &&
       fxwin <= window_cap_elems<8, false, 7, true>()) {
     rc = launch_one<MODE_ADJ, 4, 2, PSTM_F32, false, false, false, 7, true, true>(a, s);
+  } else if (a.tile_x == 4 && a.tile_y == 2 && !aa && !gen && !vti && fxwin <= window_cap_elems<8, false, 6, true>()) {
+    rc = launch_one<MODE_ADJ, 4, 2, PSTM_F32, false, false, false, 6, true>(a, s);
   } else if (a.tile_x == 4 && a.tile_y == 2 && !aa && !gen && !vti && fxwin <= window_cap_elems<8, false, 4, true>()) {
     rc = launch_one<MODE_ADJ, 4, 2, PSTM_F32, false, false, false, 4, true>(a, s);
   } else if (a.tile_x == 4 && a.tile_y == 2 && !aa && !gen && !vti &&

@@ -42,10 +42,11 @@ def test_adjoint_dot_product(flags):
     assert rel <= 1e-5
 
 
-@pytest.mark.parametrize("n_samples", [1500, 3000, 5000])
+@pytest.mark.parametrize("n_samples", [1500, 2009, 2010, 2444, 2445, 3000, 3902, 3903, 5000])
 def test_adjoint_dot_product_long_traces(n_samples):
-    # plain adjoint tiers by trace length: 4x2 tiles at 7 per CU (1D velocity,
-    # windows up to ~2000 samples), at 4 (~4000), 8x4 tiles at 2 (~6200)
+    # plain adjoint tiers by trace length and their capacity edges: 4x2 tiles
+    # at 7 per CU (1D velocity, traces up to 2009 samples), at 6 (2444), at 4
+    # (3902), 8x4 tiles at 2 (6074)
     from seisproc_amd.processors.migration.hip_ops import PstmPlan
     s = S.tiny_survey(seed=23, n_bins=8, n_classes=2, n_samples=n_samples, n_img=6, aperture=300.0)
     p = s.params.replace(n_tau=min(n_samples, 900))
