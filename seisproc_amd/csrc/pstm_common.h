@@ -113,8 +113,8 @@ constexpr int kRawBufferWord3 = 0x00020000;
 // Workgroups per CU of the kernel forms that take any admissible trace
 // length: 3 for the plain tiles and the velocity gradient, 2 for antialias.
 // Traces whose whole window fits a smaller LDS share run at higher
-// occupancy: the plain forward at 4 (7 with a velocity per output time),
-// the plain adjoint at 4 (7), the antialias forward at 4 (else 3).
+// occupancy: the plain forward and the plain adjoint at 6, then 4 (7 first
+// with a velocity per output time), the antialias forward at 4 (else 3).
 constexpr int default_occ(bool aa) { return aa ? 2 : 3; }
 // Capacity every call can rely on (8-column tiles).
 constexpr int min_cap_elems(bool aa) {
