@@ -28,6 +28,13 @@ constexpr int batch_of(int occ) { return occ >= 6 ? 16 : occ >= 4 ? 32 : 64; }
 // a time (a working set that stays in the caches; measured 5-7 % faster than
 // one launch over all offsets on the 1M-trace, 25-class workload).
 constexpr int kOffsetBands = 16;
+// Adjoint fixed-point taps: the FP32 form rounds every tap at 2^-30 of the
+// workgroup's largest per-lane bound G.  It is taken when the smallest nonzero
+// per-lane bound is at least G / kNarrowSpan, so that a lane's taps are rounded
+// at 2^-22 of its own bound or finer (the FP32 output traces round at 2^-24 of
+// each sample); wider spans (the shallowest time block under the FULL weight,
+// or image tiles with near-zero columns) take FP64 taps rounded at 2^-47 G.
+constexpr float kNarrowSpan = 256.f;
 // Traces summed in FP32 per wave before the partial sums join the running
 // (compensated) sums: a flush costs about one trace's arithmetic per column,
 // so flushing after every small sub-batch of the high-occupancy forms cost 4 %

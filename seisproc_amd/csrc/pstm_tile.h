@@ -433,10 +433,9 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
     G = fminf(G, 3.0e38f);
     int e;
     (void)frexpf(G, &e);  // G < 2^e
-    // FP32 taps rounded at 2^-30 G when the per-time bounds span at most a
-    // factor 16 (every block but the shallowest under the FULL weight); else
-    // FP64 taps rounded at 2^-47 G
-    narrow = SKEW && G <= 16.f * Gmin;
+    // FP32 taps rounded at 2^-30 G when the per-lane bounds span at most a
+    // factor kNarrowSpan; else FP64 taps rounded at 2^-47 G
+    narrow = SKEW && G <= kNarrowSpan * Gmin;
     if (narrow) {
       // taps |w m| 2^(30 - e) < 2^30: the image values carry the scale
 #pragma unroll
