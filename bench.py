@@ -214,6 +214,8 @@ def _bench_lsm(args, torch):
               flush=True)
 
     res = cgls(plan, data, vel, n_iter=args.steps, flags=0, callback=on_iteration)
+    # the rate averages the iterations after the first: the first one also
+    # carries CGLS's initial migration of the data (one more forward launch)
     stamps = stamps[1:]
     per_it = [b - a for a, b in zip(stamps, stamps[1:])]
     sec_it = sum(per_it) / len(per_it) if per_it else res.seconds_per_iteration
