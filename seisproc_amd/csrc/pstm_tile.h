@@ -233,7 +233,10 @@ __device__ __forceinline__ float taper_of(float hm2, float hl2, float tw) {
 //
 // V1 (velocity given per output time only, PSTM_VEL_1D): the velocity terms
 // are the same for every column of the tile and are held once per lane.
-template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN, int OCC, bool FIXED, bool V1>
+// WM: the weight mode compiled into the fast forms (GEN = false): PSTM_WEIGHT_FULL
+// or PSTM_WEIGHT_SPREADING (the class API's default); the general forms read it.
+template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN, int OCC, bool FIXED, bool V1,
+          int WM = PSTM_WEIGHT_FULL>
 __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   constexpr int NC = TX * TY;
   constexpr bool FX = MODE == MODE_ADJ && FIXED;
@@ -303,7 +306,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   const float idt2 = a.inv_dt * a.inv_dt;
   const float tqs = fmaxf(tq, 1e-12f) * idt2;
   [[maybe_unused]] const float vfloor = 1e-12f * idt2;
-  const int wmode = GEN ? a.weight_mode : (int)PSTM_WEIGHT_FULL;
+  const int wmode = GEN ? a.weight_mode : WM;
   [[maybe_unused]] const float aa_scale = AA ? a.aa_scale_over_dt / a.inv_dt : 0.f;
 
   float qd[NC], hl2[NC], cw[NC];
@@ -682,6 +685,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
             if constexpr (GEN) {
               if (wmode != PSTM_WEIGHT_FULL) wd = wmode == PSTM_WEIGHT_NONE ? 1.f : prs;
               if (use_taper) wd *= taper_of(hm, hl2[c], a.taper_width);
+            } else if constexpr (WM == PSTM_WEIGHT_SPREADING) {
+              wd = prs;
             }
             [[maybe_unused]] float L = 1.f, iL2 = 1.f;
             if constexpr (AA) {
@@ -1068,8 +1073,9 @@ inline int64_t tile_blocks(const TileArgs& a) {
 }
 
 template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN, int OCC = default_occ(AA),
-          bool FIXED = false, bool V1 = false>
+          bool FIXED = false, bool V1 = false, int WM = PSTM_WEIGHT_FULL>
 int launch_one(const TileArgs& a, hipStream_t s) {
+  static_assert(!GEN || WM == PSTM_WEIGHT_FULL, "the general forms read the weight mode");
   const int64_t blocks = tile_blocks(a);
   if (blocks <= 0) return PSTM_OK;
   if (blocks > 0x7fffffff) return set_error(PSTM_ERR_INVALID, "too many tiles (%lld)", (long long)blocks);
@@ -1078,7 +1084,7 @@ int launch_one(const TileArgs& a, hipStream_t s) {
   for (int band = a.band0; band < a.band1; ++band) {
     b.band0 = band;
     b.band1 = band + 1;
-    hipLaunchKernelGGL((k_tile<MODE, TX, TY, DT, AA, VTI, GEN, OCC, FIXED, V1>), dim3((unsigned)blocks), dim3(kThreads),
+    hipLaunchKernelGGL((k_tile<MODE, TX, TY, DT, AA, VTI, GEN, OCC, FIXED, V1, WM>), dim3((unsigned)blocks), dim3(kThreads),
                        0, s, b);
   }
   return PSTM_OK;
@@ -1087,6 +1093,10 @@ int launch_one(const TileArgs& a, hipStream_t s) {
 // Whether a call can use the compile-time fast path (GEN = false).
 inline bool fast_path(const pstm_params_t* p) {
   return p->weight_mode == PSTM_WEIGHT_FULL && !(p->flags & (PSTM_FLAG_TAPER | PSTM_FLAG_ZERO_OUTSIDE));
+}
+// The plain forward also has fast forms for the SPREADING weight.
+inline bool fast_path_spreading(const pstm_params_t* p) {
+  return p->weight_mode == PSTM_WEIGHT_SPREADING && !(p->flags & (PSTM_FLAG_TAPER | PSTM_FLAG_ZERO_OUTSIDE));
 }
 
 // Per-dtype forward dispatch (defined in pstm_fwd_<dtype>.hip).
@@ -1133,6 +1143,13 @@ int launch_forward_impl(const pstm_params_t* p, const PlanLayout& L, const TileA
       return launch_one<MODE_FWD, 4, 2, DT, false, false, false, 7, false, true>(a, s);
     if (win <= pair_cap_elems<6>()) return launch_one<MODE_FWD, 4, 2, DT, false, false, false, 6>(a, s);
     if (win <= pair_cap_elems<4>()) return launch_one<MODE_FWD, 4, 2, DT, false, false, false, 4>(a, s);
+  }
+  if (a.tile_x == 4 && a.tile_y == 2 && fast_path_spreading(p) && !aa && !vti) {
+    constexpr int SP = PSTM_WEIGHT_SPREADING;
+    if (p->vel_mode == PSTM_VEL_1D && win <= pair_cap_elems<7>())
+      return launch_one<MODE_FWD, 4, 2, DT, false, false, false, 7, false, true, SP>(a, s);
+    if (win <= pair_cap_elems<6>()) return launch_one<MODE_FWD, 4, 2, DT, false, false, false, 6, false, false, SP>(a, s);
+    if (win <= pair_cap_elems<4>()) return launch_one<MODE_FWD, 4, 2, DT, false, false, false, 4, false, false, SP>(a, s);
   }
   if (a.tile_x == 4 && a.tile_y == 2) return gen ? PSTM_FWD4(4, 2, true) : PSTM_FWD4(4, 2, false);
   if (a.tile_x == 8 && a.tile_y == 1) return PSTM_FWD4(8, 1, true);

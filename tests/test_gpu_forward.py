@@ -50,6 +50,8 @@ CASES = [
     ("antialias", dict(flags=FLAG_AA), dict()),
     ("vti-aa", dict(flags=FLAG_AA | FLAG_VTI, lateral=True), dict()),
     ("taper-spreading", dict(), dict(flags=FLAG_TAPER, weight_mode=WEIGHT_SPREADING)),
+    ("spreading-1d", dict(), dict(weight_mode=WEIGHT_SPREADING)),
+    ("spreading-3d-kahan", dict(lateral=True), dict(weight_mode=WEIGHT_SPREADING, flags=FLAG_KAHAN)),
     ("time-aperture-zero-outside", dict(), dict(flags=FLAG_TIME_APERTURE | FLAG_ZERO_OUTSIDE,
                                                  weight_mode=WEIGHT_NONE)),
     ("utm-coordinates", dict(utm=True), dict()),
