@@ -1144,6 +1144,11 @@ int launch_forward_impl(const pstm_params_t* p, const PlanLayout& L, const TileA
     if (win <= pair_cap_elems<6>()) return launch_one<MODE_FWD, 4, 2, DT, false, false, false, 6>(a, s);
     if (win <= pair_cap_elems<4>()) return launch_one<MODE_FWD, 4, 2, DT, false, false, false, 4>(a, s);
   }
+  // VTI without antialias: 4 per CU where the window fits (864 Gcontrib/s
+  // against 820 at 3 on 8 classes of the 1M-trace workload; at 6 per CU the
+  // inner loops spill)
+  if (a.tile_x == 4 && a.tile_y == 2 && !gen && !aa && vti && win <= sample_cap_elems<8, false, 4>())
+    return launch_one<MODE_FWD, 4, 2, DT, false, true, false, 4>(a, s);
   if (a.tile_x == 4 && a.tile_y == 2 && fast_path_spreading(p) && !aa && !vti) {
     constexpr int SP = PSTM_WEIGHT_SPREADING;
     if (p->vel_mode == PSTM_VEL_1D && win <= pair_cap_elems<7>())

@@ -49,6 +49,7 @@ CASES = [
     ("lateral-3d", dict(lateral=True), dict()),
     ("antialias", dict(flags=FLAG_AA), dict()),
     ("vti-aa", dict(flags=FLAG_AA | FLAG_VTI, lateral=True), dict()),
+    ("vti-plain", dict(flags=FLAG_VTI, lateral=True), dict()),
     ("taper-spreading", dict(), dict(flags=FLAG_TAPER, weight_mode=WEIGHT_SPREADING)),
     ("spreading-1d", dict(), dict(weight_mode=WEIGHT_SPREADING)),
     ("spreading-3d-kahan", dict(lateral=True), dict(weight_mode=WEIGHT_SPREADING, flags=FLAG_KAHAN)),
