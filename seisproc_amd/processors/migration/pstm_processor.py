@@ -1,8 +1,8 @@
 """PSTM as a processor: ``PSTMProcessor(BaseProcessor).process(SeismicData)``.
 
 The reference registers a migrator in its processor registry
-(processors/__init__.py:51,74) whose class lacks ``process`` (SURVEY of
-base_migrator.py:24), so ``BaseProcessor.from_dict`` could not build it.  This
+(processors/__init__.py:51,74) whose class lacks ``process``
+(base_migrator.py:24), so ``BaseProcessor.from_dict`` could not build it.  This
 module supplies the processor the registry implies:
 
   * input: SeismicData with traces (n_samples, n_traces), sample_rate in ms
