@@ -306,7 +306,7 @@ def _bench_lsm(args, torch):
 def main():
     args = _parse()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: the engine is measured on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is design-only", file=sys.stderr)
         sys.exit(2)
     import torch

@@ -290,7 +290,8 @@ def read_velocity_auto(filepath: Union[str, Path], **kwargs) -> VelocityModel:
     if suf in (".npy", ".npz"):
         return read_velocity_npy(path)
     if suf in (".sgy", ".segy"):
-        raise NotImplementedError("SEG-Y velocity cubes are not read by this build")
+        raise NotImplementedError("SEG-Y velocity cubes are not read here; load the cube with a SEG-Y "
+                                  "reader and pass it as a 3D VelocityModel or a device v_rms array")
     fmt = detect_velocity_format(path)
     units = {k: kwargs[k] for k in ("time_unit", "velocity_unit", "velocity_type") if k in kwargs}
     if fmt is VelocityFileFormat.ASCII_CDPTV:
