@@ -9,9 +9,11 @@ utils/dataset_indexer.py:634-703, each bin written by MigrationOutputStorage
 into ``output_dir/<bin name>`` and finalised with fold normalisation).  The
 reference keeps no checkpoint of this loop (processors/migration/
 checkpoint.py:221-611 is not wired to it); here the state after every bin is
-``output_dir/checkpoint/state.json`` plus the FP32 stack, written atomically,
-so a job stopped at any point restarts after its last completed bin and
-produces the same files as an uninterrupted run.
+``output_dir/checkpoint/state.json``, which names the FP32 stack file of the
+same generation and is replaced last (the single commit point), so a job
+stopped at any point restarts after its last completed bin and produces the
+same files as an uninterrupted run.  The job identity covers the parameters,
+binning, velocity, the trace geometry and a sample of the trace data.
 
 Data flow per bin (one GPU):
   host gather of the bin's traces (numpy array or memory map, trace-major)
@@ -71,6 +73,25 @@ class ArrayBinSource:
 
     def bin_summary(self) -> Dict[str, int]:
         return {b.name: int(self._members[b.name].size) for b in self.binning.enabled_bins}
+
+    def identity(self) -> str:
+        """Digest of the input: every source/receiver coordinate, the trace
+        array's shape and dtype, its file (path, size, mtime) when it is a
+        memory map, and a strided sample of its rows (at most 4096 rows,
+        each read whole)."""
+        h = hashlib.sha256()
+        for a in (self.sx, self.sy, self.rx, self.ry):
+            h.update(np.ascontiguousarray(a).tobytes())
+        h.update(repr((self.traces.shape, str(self.traces.dtype))).encode())
+        fn = getattr(self.traces, "filename", None)
+        if fn:
+            st = os.stat(fn)
+            h.update(repr((str(fn), st.st_size, st.st_mtime_ns)).encode())
+        n = self.traces.shape[0]
+        if n:
+            rows = np.unique(np.linspace(0, n - 1, min(n, 4096)).astype(np.int64))
+            h.update(np.ascontiguousarray(self.traces[rows]).tobytes())
+        return h.hexdigest()
 
     def iterate_bins(self) -> Iterator[Tuple[str, np.ndarray]]:
         """(bin name, trace numbers) of every enabled, non-empty bin."""
@@ -154,16 +175,21 @@ def _safe_name(name: str) -> str:
     return name.replace(" ", "_").replace("/", "-")
 
 
-def _atomic_save_npy(path: Path, arr: np.ndarray) -> None:
+def _atomic_save_npz(path: Path, **arrays: np.ndarray) -> None:
     tmp = path.with_name(path.name + ".tmp")
     with open(tmp, "wb") as fh:
-        np.save(fh, arr)
+        np.savez(fh, **arrays)
+        fh.flush()
+        os.fsync(fh.fileno())
     os.replace(tmp, path)
 
 
 def _atomic_write_text(path: Path, text: str) -> None:
     tmp = path.with_name(path.name + ".tmp")
-    tmp.write_text(text)
+    with open(tmp, "w") as fh:
+        fh.write(text)
+        fh.flush()
+        os.fsync(fh.fileno())
     os.replace(tmp, path)
 
 
@@ -191,7 +217,7 @@ class MigrationJobRunner:
                -> (image, fold); for tests of the driver logic.
     """
 
-    STATE_VERSION = 1
+    STATE_VERSION = 2
 
     def __init__(self, params: PstmParams, velocity: torch.Tensor, output_dir, *, eta: Optional[torch.Tensor] = None,
                  trace_dtype: str = "fp32", write_bins: bool = True, normalize: bool = True, stack: bool = True,
@@ -222,6 +248,8 @@ class MigrationJobRunner:
         h.update(np.ascontiguousarray(self.velocity.detach().float().cpu().numpy()).tobytes())
         if self.eta is not None:
             h.update(np.ascontiguousarray(self.eta.detach().float().cpu().numpy()).tobytes())
+        # the input itself: the trace geometry and an identity of the trace data
+        h.update(source.identity().encode())
         return h.hexdigest()
 
     def _load_state(self, key: str):
@@ -233,18 +261,28 @@ class MigrationJobRunner:
             raise RuntimeError(f"{f} belongs to a different job; remove the checkpoint directory to restart")
         img = fold = None
         if self.stack and st["completed"]:
-            img = torch.from_numpy(np.load(self.ckpt_dir / "stack_image.npy")).to(self.device)
-            fold = torch.from_numpy(np.load(self.ckpt_dir / "stack_fold.npy")).to(self.device)
+            with np.load(self.ckpt_dir / st["stack_file"], allow_pickle=False) as z:
+                img = torch.from_numpy(z["image"]).to(self.device)
+                fold = torch.from_numpy(z["fold"]).to(self.device)
         return list(st["completed"]), img, fold
 
     def _save_state(self, key: str, completed: List[str], img: Optional[torch.Tensor], fold: Optional[torch.Tensor]):
+        """``state.json`` is the single commit point: the stack of this
+        generation goes to its own file (``stack.<n completed>.npz``, image and
+        fold together), then the state naming that file replaces the old one,
+        then older generations are removed.  A stop at any point leaves a
+        state whose stack file exists and holds exactly its completed bins."""
         self.ckpt_dir.mkdir(parents=True, exist_ok=True)
+        stack_file = None
         if self.stack and img is not None:
-            _atomic_save_npy(self.ckpt_dir / "stack_image.npy", img.cpu().numpy())
-            _atomic_save_npy(self.ckpt_dir / "stack_fold.npy", fold.cpu().numpy())
+            stack_file = f"stack.{len(completed)}.npz"
+            _atomic_save_npz(self.ckpt_dir / stack_file, image=img.cpu().numpy(), fold=fold.cpu().numpy())
         _atomic_write_text(self.ckpt_dir / "state.json",
-                           json.dumps({"version": self.STATE_VERSION, "job_key": key, "completed": completed},
-                                      indent=2))
+                           json.dumps({"version": self.STATE_VERSION, "job_key": key, "completed": completed,
+                                       "stack_file": stack_file}, indent=2))
+        for old in self.ckpt_dir.glob("stack.*.npz"):
+            if old.name != stack_file:
+                old.unlink()
 
     # -- one bin on the GPU
     def _migrate_hip(self, p: PstmParams, traces: torch.Tensor, geo: torch.Tensor):

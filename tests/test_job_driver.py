@@ -127,3 +127,56 @@ def test_runner_needs_the_gpu_without_an_injected_migration(tmp_path):
     p, tr, geo = _job()
     with pytest.raises(RuntimeError, match="ROCm"):
         MigrationJobRunner(p, torch.full((p.n_tau,), 2000.0), tmp_path, device=torch.device("cpu"))
+
+
+def test_stop_inside_checkpoint_write_keeps_stack_and_state_consistent(tmp_path, monkeypatch):
+    # a stop between writing a generation's stack file and replacing state.json
+    # must leave the previous generation intact: resume adds each bin once
+    import os
+    from seisproc_amd.processors.migration import job_driver as jd
+    p, tr, geo = _job(seed=5)
+    t = create_uniform_offset_binning(0.0, 600.0, 4)
+    vel = torch.full((p.n_tau,), 2000.0)
+    mk = lambda d: MigrationJobRunner(p, vel, d, device=torch.device("cpu"), migrate_fn=_fake_migrate)
+    full = mk(tmp_path / "full").run(ArrayBinSource(tr, *geo, t))
+
+    real_replace = os.replace
+    n = {"calls": 0}
+
+    def flaky_replace(a, b):
+        # stop at the 3rd checkpoint: its stack file is written, state.json not yet replaced
+        if str(b).endswith("state.json"):
+            n["calls"] += 1
+            if n["calls"] == 3:
+                assert (tmp_path / "cut" / "checkpoint" / "stack.3.npz").exists()
+                raise OSError("injected stop before state.json is replaced")
+        return real_replace(a, b)
+
+    monkeypatch.setattr(jd.os, "replace", flaky_replace)
+    with pytest.raises(OSError, match="injected"):
+        mk(tmp_path / "cut").run(ArrayBinSource(tr, *geo, t))
+    monkeypatch.setattr(jd.os, "replace", real_replace)
+    st = json.loads((tmp_path / "cut" / "checkpoint" / "state.json").read_text())
+    assert st["completed"] == full.completed[:2] and st["stack_file"] == "stack.2.npz"
+    again = mk(tmp_path / "cut").run(ArrayBinSource(tr, *geo, t))
+    assert again.skipped == full.completed[:2]
+    assert torch.equal(again.stack_image, full.stack_image) and torch.equal(again.stack_fold, full.stack_fold)
+    assert sorted(f.name for f in (tmp_path / "cut" / "checkpoint").glob("stack.*.npz")) == ["stack.4.npz"]
+
+
+def test_checkpoint_of_other_input_data_is_refused(tmp_path):
+    # same parameters, binning and per-bin counts, different geometry or traces
+    p, tr, geo = _job(seed=6)
+    t = create_uniform_offset_binning(0.0, 600.0, 2)
+    vel = torch.full((p.n_tau,), 2000.0)
+    mk = lambda: MigrationJobRunner(p, vel, tmp_path, device=torch.device("cpu"), migrate_fn=_fake_migrate)
+    mk().run(ArrayBinSource(tr, *geo, t))
+    moved = (geo[0] + 1.0, geo[1], geo[2] + 1.0, geo[3])
+    with pytest.raises(RuntimeError, match="different job"):
+        mk().run(ArrayBinSource(tr, *moved, t))
+    other = tr.copy()
+    other[0, 0] += 1.0
+    with pytest.raises(RuntimeError, match="different job"):
+        mk().run(ArrayBinSource(other, *geo, t))
+    res = mk().run(ArrayBinSource(tr, *geo, t))
+    assert not res.completed and len(res.skipped) == 2
