@@ -125,7 +125,8 @@ int pstm_quantize_traces(const float* traces, int64_t n_traces, int32_t n_sample
  *   traces:      [n_traces][n_samples] of p->dtype
  *   trace_scale: [n_traces] FP32 or NULL (all 1)
  *   vel:         v_rms, FP32, layout p->vel_mode
- *   eta:         FP32, layout p->vel_mode, required with PSTM_FLAG_VTI, else ignored
+ *   eta:         FP32, layout p->vel_mode, required with PSTM_FLAG_VTI, else ignored;
+ *                every value must satisfy 1 + 2 eta > 0 (not checked on the device)
  *   image:       FP32 [il][xl][tau], image += sum of contributions
  *   image_comp:  FP32 [il][xl][tau] running compensation (PSTM_FLAG_KAHAN), or NULL
  *                (then the compensation is folded into image)

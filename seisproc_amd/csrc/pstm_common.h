@@ -66,7 +66,8 @@ struct TileShared {
   int32_t nb;
   float hwave[kWaves];          // max hlim^2 of each wave (over its columns and times)
   float qmin[kWaves], qmax[kWaves], tqa[kWaves], tqb[kWaves];
-  float emax;                   // max eta over the tile (VTI)
+  float emaxw[kWaves];          // per wave: max(eta, 0) over its columns and times (VTI)
+  float eminw[kWaves];          // per wave: min(eta, 0)
   float gmax[kWaves];           // adjoint: per-wave bound of one contribution
   float gmin[kWaves];           // adjoint: per-wave minimum of the per-time bounds
 };

@@ -313,7 +313,10 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   [[maybe_unused]] float k1[VTI ? NC : 1], m2[VTI ? NC : 1];
   [[maybe_unused]] float sv[MODE == MODE_GRAD ? NC : 1];  // 1/v
   [[maybe_unused]] float qa[AA ? NC : 1];                 // qd * aa_scale
-  float emax_l = 0.f;
+  // eta extremes of the tile (VTI), each clamped to 0 from its side: they
+  // bound the anelliptic leg time between sqrt(t0^2 + p / (1 + 2 max(eta, 0)))
+  // and sqrt(t0^2 + p / (1 + 2 min(eta, 0))), p = h^2 / v^2 (1 + 2 eta > 0)
+  float emax_l = 0.f, emin_l = 0.f;
   float qmin_l = 3.0e38f, qmax_l = 0.f, hmax_l = -1.f;
   unsigned cvalid = 0;  // bit c: column c of the tile is inside the image
 #pragma unroll
@@ -347,6 +350,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
       k1[c] = 1.0f + 2.0f * e;
       m2[c] = -2.0f * e;
       emax_l = fmaxf(emax_l, e);
+      emin_l = fminf(emin_l, e);
     }
     if (ok) {
       qmin_l = fminf(qmin_l, q);
@@ -366,7 +370,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   }
   {
     const float hw = wave_max(hmax_l), qn = wave_min(qmin_l), qx = wave_max(qmax_l);
-    const float em = wave_max(emax_l);
+    [[maybe_unused]] const float em = VTI ? wave_max(emax_l) : 0.f, en = VTI ? wave_min(emin_l) : 0.f;
     if (lane == 0) {
       S.hwave[wave] = hw;
       S.qmin[wave] = qn;
@@ -378,19 +382,25 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
       const float ta = a.tau0 + (float)ka * a.dtau, tbv = a.tau0 + (float)kb * a.dtau;
       S.tqa[wave] = 0.25f * ta * ta;
       S.tqb[wave] = 0.25f * tbv * tbv;
-      if (wave == 0) S.emax = 0.f;
+      if constexpr (VTI) {
+        S.emaxw[wave] = em;
+        S.eminw[wave] = en;
+      }
       if (wave == 0) S.queue_n = 0;
-    }
-    if constexpr (VTI) {
-      __syncthreads();
-      if (lane == 0) atomicMax((int*)&S.emax, __float_as_int(em));  // em >= 0
     }
   }
   __syncthreads();
   float hblk = -1.f;
 #pragma unroll
   for (int w = 0; w < kWaves; ++w) hblk = fmaxf(hblk, S.hwave[w]);
-  const float emax = VTI ? S.emax : 0.f;
+  float emax = 0.f, emin = 0.f;
+  if constexpr (VTI) {
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      emax = fmaxf(emax, S.emaxw[w]);
+      emin = fminf(emin, S.eminw[w]);
+    }
+  }
 
   // ---- accumulators
   float acc[NC], accb[NC], cmp[NC];
@@ -979,7 +989,9 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
             const float sfx = fmaxf(fabsf(g.x - X0), fabsf(g.x - X1)), sfy = fmaxf(fabsf(g.y - Y0), fabsf(g.y - Y1));
             const float rfx = fmaxf(fabsf(g.z - X0), fabsf(g.z - X1)), rfy = fmaxf(fabsf(g.w - Y0), fabsf(g.w - Y1));
             const float hs_max = fmaf(sfx, sfx, sfy * sfy), hr_max = fmaf(rfx, rfx, rfy * rfy);
-            const float ecorr = 1.0f / (1.0f + 2.0f * emax);
+            // anelliptic times: lower bound with the largest eta >= 0, upper
+            // bound with the smallest eta <= 0 (isotropic when eta >= 0)
+            const float ecorr = 1.0f / (1.0f + 2.0f * emax), ecorr_hi = 1.0f / (1.0f + 2.0f * emin);
             float tlo = 3.0e38f, thi = -1.f;
 #pragma unroll
             for (int w = 0; w < kWaves; ++w) {
@@ -987,7 +999,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               if (hs_min < hw && hr_min < hw) {
                 const float ta = fmaxf(S.tqa[w], 1e-12f), tbq = S.tqb[w] + 1e-12f;
                 const float lo = sqrtf(ta + hs_min * S.qmin[w] * ecorr) + sqrtf(ta + hr_min * S.qmin[w] * ecorr);
-                const float hi = sqrtf(tbq + hs_max * S.qmax[w]) + sqrtf(tbq + hr_max * S.qmax[w]);
+                const float hi = sqrtf(tbq + hs_max * S.qmax[w] * ecorr_hi) + sqrtf(tbq + hr_max * S.qmax[w] * ecorr_hi);
                 tlo = fminf(tlo, lo);
                 thi = fmaxf(thi, hi);
                 // wave w never reads past either end of the record (the staged

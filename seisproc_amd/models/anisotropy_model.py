@@ -59,6 +59,10 @@ class AnisotropyModel:
             logger.warning("Delta values outside typical range [-0.3, 0.3]")
         if np.any(1 + 2 * dl <= 0):
             raise ValueError("Unstable anisotropy: (1 + 2*delta) must be positive")
+        # 1 + 2 eta = (1 + 2 epsilon) / (1 + 2 delta): the anelliptic time
+        # (and the migration kernels' window bounds) need it positive
+        if np.any(1 + 2 * np.asarray(self.eta, dtype=np.float64) <= 0):
+            raise ValueError("Unstable anisotropy: (1 + 2*eta) must be positive (epsilon > -0.5)")
 
     @staticmethod
     def compute_eta(epsilon, delta):

@@ -47,6 +47,18 @@ def _need_cuda(t: torch.Tensor, name: str, dtype=None, shape=None) -> torch.Tens
     return t
 
 
+def _check_eta(eta: Optional[torch.Tensor], vel: torch.Tensor) -> None:
+    """The kernels take eta as given: the anelliptic time and its window
+    bounds need 1 + 2 eta > 0 at every sample (one device reduction and a
+    synchronisation per call)."""
+    if eta is None:
+        raise ValueError("VTI migration needs eta")
+    _need_cuda(eta, "eta", torch.float32, tuple(vel.shape))
+    lo = float(eta.min()) if eta.numel() else 0.0
+    if not lo > -0.5:
+        raise ValueError(f"eta must be > -0.5 everywhere (1 + 2 eta > 0), min is {lo}")
+
+
 class PstmPlan:
     """Device plan of one migration job (geometry binned once, reused by every
     forward / adjoint / gradient call of the job).
@@ -145,9 +157,7 @@ class PstmPlan:
         if scale is not None:
             _need_cuda(scale, "scale", torch.float32, (p.n_traces,))
         if cp.flags & FLAG_VTI:
-            if eta is None:
-                raise ValueError("VTI migration needs eta")
-            _need_cuda(eta, "eta", torch.float32, tuple(vel.shape))
+            _check_eta(eta, vel)
         if image is None:
             image = torch.zeros(self.image_shape, dtype=torch.float32, device=self.device)
         _need_cuda(image, "image", torch.float32, self.image_shape)
@@ -181,9 +191,7 @@ class PstmPlan:
         vm = self._vel_mode(vel)
         cp = self._call_params(flags, PSTM_F32, vm)
         if cp.flags & FLAG_VTI:
-            if eta is None:
-                raise ValueError("VTI migration needs eta")
-            _need_cuda(eta, "eta", torch.float32, tuple(vel.shape))
+            _check_eta(eta, vel)
         if out is None:
             out = torch.zeros((p.n_traces, p.n_samples), dtype=torch.float32, device=self.device)
         _need_cuda(out, "out", torch.float32, (p.n_traces, p.n_samples))

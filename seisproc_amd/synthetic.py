@@ -281,19 +281,21 @@ def single_offset_3d(seed: int = 2, n_bins: int = 200, n_samples: int = 1501, n_
 
 
 def precision_3d(seed: int = 3, n_bins: int = 200, n_classes: int = 25, n_samples: int = 1001,
-                      n_img: int = 400, device="cpu") -> SyntheticSurvey:
+                      n_img: int = 400, device="cpu", vt: bool = False) -> SyntheticSurvey:
     """Precision form: n_classes offset classes 100:100:2500 m, one trace per
     25 m CMP bin per class (n_bins^2 each, random azimuth), 1001 samples at
     2 ms; the 2D line's generator (5 diffractors + 3 flat reflectors, 25 Hz, v = 2000,
     noise 0.05 of peak); image n_img^2 x 1001 at 12.5 m; aperture 1500 m, 60
-    deg; compensated accumulation, no antialias."""
+    deg; compensated accumulation, no antialias.  vt: v_rms = 1500 + 600 t
+    (the single-offset form's v(t), laterally constant) instead of 2000 m/s,
+    for traces and migration alike."""
     gen = torch.Generator().manual_seed(seed)
     dt, d = 0.002, 25.0
     ext = d * (n_bins - 1)
     offsets = [100.0 * (i + 1) for i in range(n_classes)]
     src, rcv, sizes = _cmp_grid_geometry(gen, n_bins, n_bins, d, offsets)
     diff, flats = _events_line(gen, ext, ext, dt * (n_samples - 1))
-    vf = VrmsField(2000.0)
+    vf = VrmsField(1500.0, 600.0) if vt else VrmsField(2000.0)
     tr = synthesize(src.to(device), rcv.to(device), n_samples, dt, 25.0, diff, flats, vfield=vf,
                     noise=0.05, gen=gen)
     d_img = ext / (n_img - 1) if n_img > 1 else 12.5

@@ -187,6 +187,78 @@ def test_vti_closed_form_and_isotropic_limit(oracle_lib):
     assert oracle.leg_time_vti(3000.0**2, 1000.0, 2000.0, 0.15) < math.hypot(3000, 1000) / 2000
 
 
+def test_vti_negative_eta_closed_form_and_window_bounds(oracle_lib):
+    # eta < 0 (epsilon < delta) lengthens the time: h = 1000, z = 1000,
+    # v = 2000, eta = -0.1: t0^2 = 0.25, p = h^2/v^2 = 0.25,
+    # t^2 = t0^2 + p - 2 eta p^2 / (t0^2 + (1 + 2 eta) p) = 0.5 + 0.0125 / 0.45
+    t = oracle.leg_time_vti(1000.0**2, 1000.0, 2000.0, -0.1)
+    assert abs(t - math.sqrt(0.5 + 0.0125 / 0.45)) < 1e-14
+    assert t > math.hypot(1000, 1000) / 2000
+    # The kernels stage trace windows bounded by
+    #   t0^2 + p / (1 + 2 max(eta, 0)) <= t^2 <= t0^2 + p / (1 + 2 min(eta, 0));
+    # the bounds hold for every eta with 1 + 2 eta > 0 and are tight in the
+    # limits p -> 0 (both equal t0^2) and t0 -> 0 (t^2 = p / (1 + 2 eta)).
+    rng = np.random.default_rng(7)
+    for _ in range(4000):
+        h, z, v = rng.uniform(0, 6000), rng.uniform(1.0, 6000), rng.uniform(1400, 5000)
+        eta = rng.uniform(-0.49, 0.6)
+        t2 = oracle.leg_time_vti(h * h, z, v, eta) ** 2
+        t02, pp = (z / v) ** 2, (h / v) ** 2
+        lo, hi = t02 + pp / (1 + 2 * max(eta, 0.0)), t02 + pp / (1 + 2 * min(eta, 0.0))
+        assert lo * (1 - 1e-12) <= t2 <= hi * (1 + 1e-12), (h, z, v, eta)
+    # a window bounded with the isotropic time (the bound for eta >= 0 only)
+    # misses the far-offset time of a negative eta by far more than the
+    # kernels' 2-sample margin: at 45 deg legs, tau = 2 s and 2 ms sampling
+    z, v = 2000.0, 2000.0
+    t_iso = math.hypot(z, z) / v
+    t_neg = oracle.leg_time_vti(z * z, z, v, -0.1)
+    assert (t_neg - t_iso) / 0.002 > 5
+    t_hi = math.sqrt((z / v) ** 2 + (z / v) ** 2 / (1 - 0.2))
+    assert t_neg <= t_hi
+
+
+def test_antialias_half_width_is_the_dip_by_central_differences(oracle_lib):
+    # L = aa_scale (|dt/d il| + |dt/d xl|) / dt, the traveltime slopes per
+    # image grid index, against central differences of the DSR time of the
+    # same pair with the image point moved along each grid axis.  A rotated,
+    # anisotropic grid and asymmetric legs, so that dropping a leg, swapping
+    # the axes or adding the slopes before the absolute value all change L.
+    ang = math.radians(30.0)
+    d_il, d_xl = 12.5, 20.0
+    grid = dict(il_dx=d_il * math.cos(ang), il_dy=d_il * math.sin(ang),
+                xl_dx=-d_xl * math.sin(ang), xl_dy=d_xl * math.cos(ang))
+    aa_scale = 0.7
+    p_aa = _p(n_samples=40001, flags=FLAG_AA, aa_scale=aa_scale, aa_max=256.0, **grid)
+    p_time = _p(n_samples=40001, **grid)
+    rng = np.random.default_rng(11)
+    n_checked = n_power = 0
+    worst = 0.0
+    for _ in range(400):
+        dxs, dys, dxr, dyr = rng.uniform(-1500, 1500, 4)
+        v = rng.uniform(1500, 4000)
+        tau = rng.uniform(0.3, 3.0)
+        ok, w, u, L = oracle.pair(p_aa, dxs, dys, dxr, dyr, tau, v)
+        if not ok or not (1.5 < L < 250):
+            continue
+
+        def u_at(ex, ey, h):
+            # image point moved by h grid steps: x - s and x - r both grow by h e
+            return oracle.pair(p_time, dxs + h * ex, dys + h * ey, dxr + h * ex, dyr + h * ey, tau, v)[2]
+
+        h = 1e-3
+        s_il = (u_at(grid["il_dx"], grid["il_dy"], h) - u_at(grid["il_dx"], grid["il_dy"], -h)) / (2 * h)
+        s_xl = (u_at(grid["xl_dx"], grid["xl_dy"], h) - u_at(grid["xl_dx"], grid["xl_dy"], -h)) / (2 * h)
+        L_fd = aa_scale * (abs(s_il) + abs(s_xl))
+        worst = max(worst, abs(L - L_fd) / L_fd)
+        # plausible mistakes, each off by far more than the FD error
+        wrong = [aa_scale * abs(s_il + s_xl), aa_scale * (abs(s_il) * d_xl / d_il + abs(s_xl) * d_il / d_xl)]
+        n_checked += 1
+        if all(abs(x - L_fd) / L_fd > 0.02 for x in wrong):
+            n_power += 1
+    assert worst < 1e-6
+    assert n_checked > 100 and n_power > 30
+
+
 def test_vti_forward_equals_isotropic_at_eta_zero(oracle_lib):
     s = S.tiny_survey(seed=3, n_bins=6, n_classes=2, n_samples=151, n_img=4, aperture=300.0)
     sx, sy, rx, ry = s.numpy_geometry()
