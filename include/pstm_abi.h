@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define PSTM_ABI_VERSION 1
+#define PSTM_ABI_VERSION 2
 
 typedef enum {
   PSTM_OK = 0,
@@ -48,7 +48,7 @@ enum {
   PSTM_FLAG_AA = 1u << 0,            /* triangle antialias filter on integrated traces */
   PSTM_FLAG_VTI = 1u << 1,           /* per-leg anelliptic eta traveltime */
   PSTM_FLAG_KAHAN = 1u << 2,         /* compensated FP32 accumulation */
-  PSTM_FLAG_TAPER = 1u << 3,         /* cosine taper towards the aperture/angle limit */
+  PSTM_FLAG_TAPER = 1u << 3,         /* cosine edge tapers: distance and angle of both legs, offset window */
   PSTM_FLAG_ZERO_OUTSIDE = 1u << 4,  /* drop contributions outside the record (default: clamp) */
   PSTM_FLAG_TIME_APERTURE = 1u << 5  /* aperture(z) = min(aperture_m, z*tan(max_angle)) */
 };
@@ -80,13 +80,20 @@ typedef struct {
   double xl_dx, xl_dy; /* world step per crossline index (must be orthogonal to the inline step) */
   double aperture_m;   /* maximum source/receiver-to-image horizontal distance */
   double max_angle_deg;/* maximum angle from vertical, per leg */
-  double taper_width;  /* fraction of the limit used by PSTM_FLAG_TAPER, in (0, 1) */
+  double taper_width;  /* fraction of each limit tapered by PSTM_FLAG_TAPER, in (0, 1) */
   double aa_scale;     /* triangle half-width multiplier (PSTM_FLAG_AA) */
   double aa_max;       /* maximum triangle half-width in samples (PSTM_FLAG_AA) */
   int32_t dtype;       /* pstm_dtype_t of the traces */
   int32_t flags;       /* PSTM_FLAG_* */
   int32_t weight_mode; /* PSTM_WEIGHT_* */
   int32_t vel_mode;    /* PSTM_VEL_* (also the layout of eta) */
+  /* Offset window (source-receiver distance), a property of the plan:
+   * with max_offset_m > 0 pstm_plan_build keeps only traces with
+   * min_offset_m <= offset <= max_offset_m (reference
+   * processors/migration/aperture.py:184-224), and PSTM_FLAG_TAPER adds
+   * its near and far cosine tapers.  max_offset_m <= 0: no window. */
+  double min_offset_m;
+  double max_offset_m;
 } pstm_params_t;
 
 /* Version of this ABI (PSTM_ABI_VERSION). */

@@ -23,6 +23,27 @@ def _lerp(a: np.ndarray, x: float) -> float:
     return a[i] + f * (a[i + 1] - a[i])
 
 
+def _edge(x: float, xmax: float, tw: float) -> float:
+    """Cosine edge taper (processors/migration/aperture.py:226-270)."""
+    s = xmax * (1 - tw)
+    if x > xmax:
+        return 0.0
+    return 0.5 * (1 + np.cos(np.pi * (x - s) / (xmax - s))) if x > s else 1.0
+
+
+def _offset_taper(p: dict, off: float) -> float:
+    """Offset window with near/far cosine tapers (aperture.py:184-224)."""
+    lo, hi = p.get("min_offset_m", 0.0), p.get("max_offset_m", 0.0)
+    if not hi > 0:
+        return 1.0
+    if not lo <= off <= hi:
+        return 0.0
+    tr = p["taper_width"] * (hi - lo)
+    near = 0.5 * (1 - np.cos(np.pi * (off - lo) / tr)) if off < lo + tr else 1.0
+    far = 0.5 * (1 + np.cos(np.pi * (off - (hi - tr)) / tr)) if off > hi - tr else 1.0
+    return near * far
+
+
 def forward(p: dict, traces, sx, sy, rx, ry, vel, eta=None):
     n_il, n_xl, nt, ns = p["n_il"], p["n_xl"], p["n_tau"], p["n_samples"]
     flags = p.get("flags", 0)
@@ -34,11 +55,16 @@ def forward(p: dict, traces, sx, sy, rx, ry, vel, eta=None):
     fold = np.zeros((n_il, n_xl, nt))
     vel = np.asarray(vel, dtype=np.float64)
     aint = [_integrate(np.asarray(traces[j], np.float64), pad) for j in range(len(sx))] if aa else None
+    off = np.hypot(np.asarray(rx) - np.asarray(sx), np.asarray(ry) - np.asarray(sy))
+    lo, hi = p.get("min_offset_m", 0.0), p.get("max_offset_m", 0.0)
+    kept = (off >= lo) & (off <= hi) if hi > 0 else np.ones(len(sx), bool)
     for il in range(n_il):
         for xl in range(n_xl):
             x = p["origin_x"] + il * p["il_dx"] + xl * p["xl_dx"]
             y = p["origin_y"] + il * p["il_dy"] + xl * p["xl_dy"]
             for j in range(len(sx)):
+                if not kept[j]:
+                    continue
                 dxs, dys, dxr, dyr = x - sx[j], y - sy[j], x - rx[j], y - ry[j]
                 hs, hr = np.hypot(dxs, dys), np.hypot(dxr, dyr)
                 for k in range(nt):
@@ -66,11 +92,9 @@ def forward(p: dict, traces, sx, sy, rx, ry, vel, eta=None):
                          1: 1.0 / (rs * rr + EPS), 2: 1.0,
                          3: (z / (rs + EPS)) * (z / (rr + EPS))}[wm]
                     if taper:
-                        lim = min(ap, np.tan(th) * (z + EPS))
-                        rho = max(hs, hr) / lim
-                        st = 1 - p["taper_width"]
-                        if rho > st:
-                            w *= 0.5 * (1 + np.cos(np.pi * min((rho - st) / p["taper_width"], 1.0)))
+                        tw = p["taper_width"]
+                        w *= (_edge(hs, ap, tw) * _edge(hr, ap, tw) * _edge(np.arctan2(hs, abs(z)), th, tw)
+                              * _edge(np.arctan2(hr, abs(z)), th, tw) * _offset_taper(p, off[j]))
                     u = (t - p["t0"]) / p["dt"]
                     if zo and (u < 0 or u > ns - 1):
                         continue

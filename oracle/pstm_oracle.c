@@ -22,7 +22,9 @@
  *     models/migration_config.py:40-45)
  *   processors/migration/traveltime_vti.py:164-222    anelliptic eta time per
  *     leg (v0 = v_rms, delta = 0, epsilon = eta)
- *   processors/migration/weights.py:192-236           cosine taper shape
+ *   processors/migration/aperture.py:147-270          taper composition (distance
+ *     and angle tapers of both legs, weights.py:192-236 shape) and the
+ *     offset window with near/far tapers (:184-224)
  *
  * Extensions with no reference code (defined here, pinned by the tests):
  *   - v_rms(il, xl, tau): the reference's constant v generalised per image
@@ -31,7 +33,6 @@
  *     the causal-then-anticausal double integral of the zero-extended trace,
  *     linearly interpolated; L = clamp(aa_scale (|dt/dil| + |dt/dxl|) / dt,
  *     1, aa_max) samples.  L = 1 reproduces plain linear interpolation;
- *   - taper on rho = max(h_s, h_r) / limit with the weights.py cosine shape;
  *   - exact adjoint (transpose) and the analytic velocity gradient with the
  *     masks held at `mask_vel`.
  * Parity pins: tests/test_oracle_*.py (known answers restated from the
@@ -85,21 +86,49 @@ static inline int mask_pass(const ctx_t* c, double hs, double hr, double zm) {
   return 1;
 }
 
-static inline double taper_value(const ctx_t* c, double hs, double hr, double zm) {
+/* Cosine edge taper of x towards its limit xmax
+ * (processors/migration/aperture.py:226-270, same shape as weights.py:192-236):
+ * 1 for x <= s = xmax (1 - tw), 0.5 (1 + cos(pi (x - s) / (xmax - s))) on
+ * (s, xmax], 0 beyond xmax. */
+static inline double edge_taper(double x, double xmax, double tw) {
+  double s = xmax * (1.0 - tw);
+  if (x > xmax) return 0.0;
+  if (x > s) return 0.5 * (1.0 + cos(M_PI * (x - s) / (xmax - s)));
+  return 1.0;
+}
+
+/* Offset window with near and far cosine tapers
+ * (processors/migration/aperture.py:184-224); 1 when no window is set. */
+static inline double offset_taper(const pstm_params_t* p, double off) {
+  if (!(p->max_offset_m > 0)) return 1.0;
+  double lo = p->min_offset_m, hi = p->max_offset_m;
+  if (!(off >= lo && off <= hi)) return 0.0;
+  double tr = p->taper_width * (hi - lo);
+  double near = (off < lo + tr) ? 0.5 * (1.0 - cos(M_PI * (off - lo) / tr)) : 1.0;
+  double far = (off > hi - tr) ? 0.5 * (1.0 + cos(M_PI * (off - (hi - tr)) / tr)) : 1.0;
+  return near * far;
+}
+
+/* Inside the offset window of the plan (all offsets without one). */
+static inline int offset_kept(const pstm_params_t* p, double off) {
+  return !(p->max_offset_m > 0) || (off >= p->min_offset_m && off <= p->max_offset_m);
+}
+
+/* Taper composition of processors/migration/aperture.py:147-182: distance
+ * tapers of both legs towards the aperture (the time-dependent aperture under
+ * PSTM_FLAG_TIME_APERTURE), angle tapers of both legs with angle
+ * atan2(h, |z|) towards max_angle, and the trace's offset taper. */
+static inline double taper_value(const ctx_t* c, double hs, double hr, double zm, double off) {
   const pstm_params_t* p = c->p;
-  double lim = p->aperture_m;
+  double ap = p->aperture_m;
   if (p->flags & PSTM_FLAG_TIME_APERTURE) {
     double a2 = zm * c->tan_th;
-    if (a2 < lim) lim = a2;
+    if (a2 < ap) ap = a2;
   }
-  double al = c->tan_th * (zm + EPS_R);
-  if (al < lim) lim = al;
-  double rho = (hs > hr ? hs : hr) / lim;
   double tw = p->taper_width;
-  double start = 1.0 - tw;
-  if (rho <= start) return 1.0;
-  if (rho >= 1.0) return 0.0;
-  return 0.5 * (1.0 + cos(M_PI * (rho - start) / tw));
+  double t = edge_taper(hs, ap, tw) * edge_taper(hr, ap, tw);
+  t *= edge_taper(atan2(hs, fabs(zm)), c->th_rad, tw) * edge_taper(atan2(hr, fabs(zm)), c->th_rad, tw);
+  return t * offset_taper(p, off);
 }
 
 /* traveltime_vti.py:164-222 with v0 = v_nmo = v (delta = 0), eta given. */
@@ -152,7 +181,7 @@ static int eval_pair(const ctx_t* c, double dxs, double dys, double dxr, double 
     w = 1.0;
   double tap = 1.0;
   if (p->flags & PSTM_FLAG_TAPER) {
-    tap = taper_value(c, hs, hr, zm);
+    tap = taper_value(c, hs, hr, zm, hypot(dxs - dxr, dys - dyr));
     w *= tap;
   }
   double n = p->n_samples;
@@ -320,6 +349,7 @@ int oracle_forward(const pstm_params_t* p, const double* traces, const double* s
       double* img = image + (size_t)col * nt;
       double* fld = fold ? fold + (size_t)col * nt : NULL;
       for (int64_t j = 0; j < ntr; ++j) {
+        if (!offset_kept(p, hypot(rx[j] - sx[j], ry[j] - sy[j]))) continue;
         double dxs = x - sx[j], dys = y - sy[j], dxr = x - rx[j], dyr = y - ry[j];
         double hs = sqrt(dxs * dxs + dys * dys), hr = sqrt(dxr * dxr + dyr * dyr);
         if (!(hs < p->aperture_m && hr < p->aperture_m)) continue;
@@ -365,6 +395,7 @@ int oracle_adjoint(const pstm_params_t* p, const double* image, const double* sx
 #pragma omp for schedule(dynamic, 1)
     for (int64_t j = 0; j < ntr; ++j) {
       double* dout = traces_out + j * n;
+      if (!offset_kept(p, hypot(rx[j] - sx[j], ry[j] - sy[j]))) continue;
       if (do_aa) memset(abar, 0, sizeof(double) * ne);
       for (int col = 0; col < ncol; ++col) {
         int il = col / p->n_xl, xl = col % p->n_xl;
@@ -450,6 +481,7 @@ int oracle_vel_grad(const pstm_params_t* p, const double* traces, const double* 
       const double* g = image_grad + (size_t)col * nt;
       double* out = vgrad + (size_t)col * nt;
       for (int64_t j = 0; j < ntr; ++j) {
+        if (!offset_kept(p, hypot(rx[j] - sx[j], ry[j] - sy[j]))) continue;
         double dxs = x - sx[j], dys = y - sy[j], dxr = x - rx[j], dyr = y - ry[j];
         double hs = sqrt(dxs * dxs + dys * dys), hr = sqrt(dxr * dxr + dyr * dyr);
         if (!(hs < p->aperture_m && hr < p->aperture_m)) continue;

@@ -7,7 +7,7 @@ from __future__ import annotations
 import ctypes
 from typing import Any, Dict
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 # pstm_status_t
 PSTM_OK = 0
@@ -66,13 +66,16 @@ class PstmParams(ctypes.Structure):
         ("flags", ctypes.c_int32),
         ("weight_mode", ctypes.c_int32),
         ("vel_mode", ctypes.c_int32),
+        ("min_offset_m", ctypes.c_double),
+        ("max_offset_m", ctypes.c_double),
     ]
 
     DEFAULTS: Dict[str, Any] = dict(
         n_traces=0, n_samples=0, n_il=1, n_xl=1, n_tau=1, dt=0.002, t0=0.0, dtau=0.002,
         tau0=0.0, origin_x=0.0, origin_y=0.0, il_dx=25.0, il_dy=0.0, xl_dx=0.0, xl_dy=25.0,
         aperture_m=5000.0, max_angle_deg=60.0, taper_width=0.1, aa_scale=1.0, aa_max=16.0,
-        dtype=PSTM_F32, flags=0, weight_mode=WEIGHT_FULL, vel_mode=VEL_1D)
+        dtype=PSTM_F32, flags=0, weight_mode=WEIGHT_FULL, vel_mode=VEL_1D, min_offset_m=0.0,
+        max_offset_m=0.0)
 
     @classmethod
     def from_dict(cls, d: Dict[str, Any]) -> "PstmParams":

@@ -60,6 +60,11 @@ static int validate(const pstm_params_t* p) {
   if (p->flags & ~known) return set_error(PSTM_ERR_INVALID, "unknown flag bits 0x%x", p->flags & ~known);
   if ((p->flags & PSTM_FLAG_TAPER) && !(p->taper_width > 0 && p->taper_width < 1))
     return set_error(PSTM_ERR_INVALID, "taper_width must be in (0, 1)");
+  if (!(p->min_offset_m >= 0) || !(p->max_offset_m == p->max_offset_m))
+    return set_error(PSTM_ERR_INVALID, "min_offset_m must be >= 0 and max_offset_m a number");
+  if (p->max_offset_m > 0 && !(p->min_offset_m < p->max_offset_m))
+    return set_error(PSTM_ERR_INVALID, "min_offset_m (%g) must be below max_offset_m (%g)", p->min_offset_m,
+                     p->max_offset_m);
   const double dil = hypot(p->il_dx, p->il_dy), dxl = hypot(p->xl_dx, p->xl_dy);
   if (!(dil > 0) || !(dxl > 0)) return set_error(PSTM_ERR_INVALID, "grid steps must be non-zero");
   if (fabs(p->il_dx * p->xl_dx + p->il_dy * p->xl_dy) > 1e-6 * dil * dxl)
@@ -121,6 +126,8 @@ int compute_layout(const pstm_params_t* p, PlanLayout* L) {
   l.off_ids_out = off;    off = align_up(off + 4 * n);
   l.off_local = off;      off = align_up(off + 16 * n);
   l.off_geom = off;       off = align_up(off + 16 * n);
+  l.off_offs_local = off; off = align_up(off + 4 * n);
+  l.off_offs = off;       off = align_up(off + 4 * n);
   l.off_cell_start = off; off = align_up(off + 4 * ((size_t)n_cells + 1));
   l.sort_tmp_bytes = n > 0 ? sort_temp_bytes((int64_t)n, bits) : 0;
   l.off_sort_tmp = off;   off = align_up(off + l.sort_tmp_bytes);
@@ -162,6 +169,9 @@ void fill_tile_args(const pstm_params_t* p, const PlanLayout& L, const void* ws,
   t.taper_width = (float)p->taper_width;
   t.aa_scale_over_dt = (float)(p->aa_scale / p->dt);
   t.aa_max = (float)p->aa_max;
+  t.th_max = (float)(p->max_angle_deg * M_PI / 180.0);
+  t.off_min = (float)p->min_offset_m;
+  t.off_max = p->max_offset_m > 0 ? (float)p->max_offset_m : 0.f;
   t.aa_pad = (p->flags & PSTM_FLAG_AA) ? (int)ceil(p->aa_max) + 2 : 0;
   t.flags = p->flags;
   t.weight_mode = p->weight_mode;
@@ -179,6 +189,7 @@ void fill_tile_args(const pstm_params_t* p, const PlanLayout& L, const void* ws,
   t.geom = (const float4*)(base + L.off_geom);
   t.ids = (const int32_t*)(base + L.off_ids_out);
   t.cell_start = (const int32_t*)(base + L.off_cell_start);
+  t.offs = (const float*)(base + L.off_offs);
   *a = t;
 }
 

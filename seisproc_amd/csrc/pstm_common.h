@@ -153,6 +153,7 @@ struct PlanLayout {
   // byte offsets inside the workspace
   size_t off_header, off_keys_in, off_keys_out, off_ids_in, off_ids_out;
   size_t off_local, off_geom, off_cell_start, off_sort_tmp;
+  size_t off_offs_local, off_offs;  // per-trace offsets (FP32 of the FP64 distance), unsorted / sorted
   size_t sort_tmp_bytes;
   size_t total_bytes;
 };
@@ -179,6 +180,8 @@ struct TileArgs {
   float aperture, tan_th;
   float taper_width;
   float aa_scale_over_dt, aa_max;
+  float th_max;             // max_angle in radians (taper)
+  float off_min, off_max;   // offset window (taper; off_max <= 0: none)
   int32_t aa_pad;           // staged zero padding on each side (samples)
   int32_t flags, weight_mode, vel_mode;
   // plan
@@ -188,6 +191,7 @@ struct TileArgs {
   const float4* geom;        // sorted local (sa, sb, ra, rb)
   const int32_t* ids;        // sorted -> original trace index
   const int32_t* cell_start; // [n_bands*ncx*ncy + 1]
+  const float* offs;         // sorted source-receiver offsets (m)
   // data
   const void* traces;
   const float* scale;

@@ -30,7 +30,8 @@ __global__ void k_plan_local(const double* __restrict__ src, const double* __res
                              double uy_x, double uy_y, double lo_a, double hi_a, double lo_b,
                              double hi_b, double cx0, double cy0, double cw, double ch,
                              int32_t ncx, int32_t ncy, int32_t n_bands, double two_a,
-                             float4* __restrict__ local, uint32_t* __restrict__ keys,
+                             double omin, double omax, float4* __restrict__ local,
+                             float* __restrict__ offs, uint32_t* __restrict__ keys,
                              int32_t* __restrict__ ids) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
@@ -44,8 +45,11 @@ __global__ void k_plan_local(const double* __restrict__ src, const double* __res
   // A trace can only contribute if its midpoint is within the aperture of
   // some image column (|m - x| <= (h_s + h_r)/2 < aperture) and its offset is
   // below twice the aperture (h_s + h_r >= offset).
+  // With an offset window (omax > 0) only traces inside [omin, omax] are
+  // kept (processors/migration/aperture.py:184-224, decided in FP64).
   const double off = hypot(sa - ra, sb - rb);
-  if (ma > lo_a && ma < hi_a && mb > lo_b && mb < hi_b && off < two_a) {
+  const bool in_window = !(omax > 0) || (off >= omin && off <= omax);
+  if (ma > lo_a && ma < hi_a && mb > lo_b && mb < hi_b && off < two_a && in_window) {
     int32_t cx = (int32_t)floor((ma - cx0) / cw);
     int32_t cy = (int32_t)floor((mb - cy0) / ch);
     cx = cx < 0 ? 0 : (cx >= ncx ? ncx - 1 : cx);
@@ -55,15 +59,19 @@ __global__ void k_plan_local(const double* __restrict__ src, const double* __res
     key = (uint32_t)band * n_cells + (uint32_t)cy * (uint32_t)ncx + (uint32_t)cx;
   }
   local[j] = make_float4((float)sa, (float)sb, (float)ra, (float)rb);
+  offs[j] = (float)off;
   keys[j] = key;
   ids[j] = (int32_t)j;
 }
 
-__global__ void k_plan_gather(const float4* __restrict__ local, const int32_t* __restrict__ ids,
-                              int64_t n, float4* __restrict__ geom) {
+__global__ void k_plan_gather(const float4* __restrict__ local, const float* __restrict__ offs_in,
+                              const int32_t* __restrict__ ids, int64_t n, float4* __restrict__ geom,
+                              float* __restrict__ offs) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  geom[i] = local[ids[i]];
+  const int32_t j = ids[i];
+  geom[i] = local[j];
+  offs[i] = offs_in[j];
 }
 
 // cell_start[c] = first sorted index with key >= c, for c in [0, n_cells]
@@ -100,6 +108,8 @@ int plan_build(const pstm_params_t* p, const PlanLayout& L, const double* src_xy
   auto* local = (float4*)(base + L.off_local);
   auto* geom = (float4*)(base + L.off_geom);
   auto* cells = (int32_t*)(base + L.off_cell_start);
+  auto* offs_local = (float*)(base + L.off_offs_local);
+  auto* offs = (float*)(base + L.off_offs);
   const int64_t n = L.n_traces;
   const uint32_t n_cells = (uint32_t)L.ncx * (uint32_t)L.ncy * (uint32_t)L.n_bands;  // sort keys
 
@@ -111,13 +121,15 @@ int plan_build(const pstm_params_t* p, const PlanLayout& L, const double* src_xy
     hipLaunchKernelGGL(k_plan_local, dim3(blocks), dim3(256), 0, s, src_xy, rcv_xy, n,
                        p->origin_x, p->origin_y, L.ux_x, L.ux_y, L.uy_x, L.uy_y, -A, W + A, -A,
                        H + A, (double)L.cell_x0, (double)L.cell_y0, (double)L.cell_w,
-                       (double)L.cell_h, L.ncx, L.ncy, L.n_bands, 2.0 * A, local, keys_in, ids_in);
+                       (double)L.cell_h, L.ncx, L.ncy, L.n_bands, 2.0 * A, p->min_offset_m,
+                       p->max_offset_m, local, offs_local, keys_in, ids_in);
     size_t tmp_bytes = L.sort_tmp_bytes;
     hipError_t e = hipcub::DeviceRadixSort::SortPairs(base + L.off_sort_tmp, tmp_bytes, keys_in,
                                                       keys_out, ids_in, ids_out, (int)n, 0,
                                                       L.key_bits, s);
     if (e != hipSuccess) return set_hip_error("radix sort", e);
-    hipLaunchKernelGGL(k_plan_gather, dim3(blocks), dim3(256), 0, s, local, ids_out, n, geom);
+    hipLaunchKernelGGL(k_plan_gather, dim3(blocks), dim3(256), 0, s, local, offs_local, ids_out, n, geom,
+                       offs);
   }
   int cblocks = (int)((n_cells + 1 + 255) / 256);
   hipLaunchKernelGGL(k_plan_cells, dim3(cblocks), dim3(256), 0, s, keys_out, n, n_cells, cells,

@@ -195,13 +195,41 @@ __device__ __forceinline__ void two_sum_add(float& s, float& c, float y) {
   s = t;
 }
 
-// Cosine taper on rho = h / limit (weights.py:192-236 shape).
-__device__ __forceinline__ float taper_of(float hm2, float hl2, float tw) {
-  float rho = sqrtf(hm2 / hl2);
-  float start = 1.0f - tw;
-  if (rho <= start) return 1.0f;
-  float x = (rho - start) / tw;
-  return 0.5f * (1.0f + __cosf(3.14159265358979f * fminf(x, 1.0f)));
+// Cosine edge taper of x towards its limit xmax (processors/migration/
+// aperture.py:226-270, the shape of weights.py:192-236): 1 up to
+// s = xmax (1 - tw), 0.5 (1 + cos(pi (x - s) / (xmax - s))) on (s, xmax],
+// 0 beyond (selects: a zero-width zone never reaches the division's NaN).
+__device__ __forceinline__ float edge_taper(float x, float xmax, float tw) {
+  const float s = xmax * (1.0f - tw);
+  const float c = 0.5f * (1.0f + __cosf(3.14159265358979f * (x - s) / (xmax - s)));
+  return x > xmax ? 0.f : (x > s ? c : 1.f);
+}
+
+// Taper composition of processors/migration/aperture.py:147-182: distance
+// tapers of both legs towards the aperture (time-dependent under
+// PSTM_FLAG_TIME_APERTURE) times angle tapers of both legs, angle
+// atan2(h, z).  hs2, hr2: squared leg distances (m^2); z = v tau / 2 (m).
+// The trace's offset taper multiplies its staged samples (forward, gradient)
+// or its flushed window (adjoint) instead.
+__device__ __forceinline__ float aperture_taper(float hs2, float hr2, float z, const TileArgs& a) {
+  float ap = a.aperture;
+  if (a.flags & PSTM_FLAG_TIME_APERTURE) ap = fminf(ap, z * a.tan_th);
+  const float hs = sqrtf(hs2), hr = sqrtf(hr2), tw = a.taper_width;
+  return edge_taper(hs, ap, tw) * edge_taper(hr, ap, tw) * edge_taper(atan2f(hs, z), a.th_max, tw) *
+         edge_taper(atan2f(hr, z), a.th_max, tw);
+}
+
+// Offset window taper of one trace (aperture.py:184-224): near taper
+// 0.5 (1 - cos(pi (o - min) / r)) below min + r, far taper
+// 0.5 (1 + cos(pi (o - (max - r)) / r)) above max - r, r = tw (max - min).
+// The plan keeps only traces inside [min, max]; 1 without a window.
+__device__ __forceinline__ float offset_taper(float off, const TileArgs& a) {
+  if (!(a.off_max > 0.f)) return 1.f;
+  const float r = a.taper_width * (a.off_max - a.off_min);
+  const float pi = 3.14159265358979f;
+  const float nr = off < a.off_min + r ? 0.5f * (1.0f - __cosf(pi * (off - a.off_min) / r)) : 1.f;
+  const float fr = off > a.off_max - r ? 0.5f * (1.0f + __cosf(pi * (off - (a.off_max - r)) / r)) : 1.f;
+  return nr * fr;
 }
 
 // GEN = false: FULL weight, no taper, clamped record end (compile-time);
@@ -542,7 +570,9 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
             }
           } else {
             const int id = a.ids[S.q_j[e]];
-            const float sc = a.scale ? a.scale[id] : 1.f;
+            // per-trace factors: the storage scale, and the offset taper (the
+            // operator is linear in the trace samples)
+            const float sc = (a.scale ? a.scale[id] : 1.f) * (use_taper ? offset_taper(a.offs[S.q_j[e]], a) : 1.f);
             const int64_t row = (int64_t)id * a.n_samples;
             if constexpr (PAIR) {
 #pragma clang fp contract(off)
@@ -694,7 +724,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
             float wd = prs * prs;
             if constexpr (GEN) {
               if (wmode != PSTM_WEIGHT_FULL) wd = wmode == PSTM_WEIGHT_NONE ? 1.f : prs;
-              if (use_taper) wd *= taper_of(hm, hl2[c], a.taper_width);
+              if (use_taper) wd *= aperture_taper(h2.x, h2.y, sqrtf(tqs / qd[c]), a);
             } else if constexpr (WM == PSTM_WEIGHT_SPREADING) {
               wd = prs;
             }
@@ -931,6 +961,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
             const int w0 = S.q_w0[e], len = S.q_len[e], off = S.b_off[b];
             const int id = a.ids[S.q_j[e]];
             float* row = a.traces_out + (int64_t)id * a.n_samples;
+            // transpose of the forward's per-trace offset taper
+            const float ow = use_taper ? offset_taper(a.offs[S.q_j[e]], a) : 1.f;
             [[maybe_unused]] const long long* const Lw = Wl + S.b_base[b];
             for (int s = lane; s < len; s += 64) {
               const int dst = w0 + s;
@@ -942,7 +974,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               } else if constexpr (FX && AA) v = (float)(Wa[off + s] * fx_inv);
               else if constexpr (FX) v = (float)(from_fixed64(Wl[off + s]) * fx_inv);
               else v = AA ? (float)Wa[off + s] : Wf[off + s];
-              if (dst >= 0 && dst < a.n_samples && v != 0.f) atomicAdd(&row[dst], v);
+              if (dst >= 0 && dst < a.n_samples && v != 0.f) atomicAdd(&row[dst], v * ow);
             }
           }
           __syncthreads();

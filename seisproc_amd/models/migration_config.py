@@ -222,6 +222,12 @@ class MigrationConfig:
                        time_dependent_aperture: bool = False, taper: bool = False):
         """C-ABI job description for this configuration.
 
+        The offset window [min_offset_m, max_offset_m] is a property of the
+        plan (traces outside it are dropped); ``taper`` turns on the
+        reference's taper composition (processors/migration/aperture.py:
+        147-270: distance and angle cosine tapers of both legs and the offset
+        window's near/far tapers, each over taper_width of its range).
+
         Weight modes map one to one (FULL = cos*cos/(r*r) of
         kirchhoff_kernel.py:691-693, SPREADING = 1/(r_s r_r) of
         kirchhoff_migrator.py:378).  The traveltime is the straight-ray DSR,
@@ -255,7 +261,8 @@ class MigrationConfig:
             aperture_m=self.max_aperture_m, max_angle_deg=self.max_angle_deg,
             taper_width=self.taper_width if 0 < self.taper_width < 1 else 0.1,
             aa_scale=aa_scale, aa_max=aa_max, flags=flags,
-            weight_mode=WEIGHT_CODES[self.weight_mode.value]))
+            weight_mode=WEIGHT_CODES[self.weight_mode.value],
+            min_offset_m=self.min_offset_m, max_offset_m=self.max_offset_m))
 
 
 def create_default_config(n_time: int = 1000, n_inline: int = 100, n_xline: int = 100,

@@ -28,7 +28,9 @@ def _setup(flags=0, lateral=True, seed=20, **kw):
 
 @pytest.mark.parametrize("flags", [0, FLAG_AA, FLAG_AA | FLAG_VTI | FLAG_TAPER])
 def test_adjoint_dot_product(flags):
-    s, plan, p, vel = _setup(flags)
+    # with the taper, also the offset window's near and far tapers
+    win = dict(min_offset_m=100.0, max_offset_m=500.0, taper_width=0.3) if flags & FLAG_TAPER else {}
+    s, plan, p, vel = _setup(flags, **win)
     eta = s.eta.to(DEV) if s.eta is not None else None
     g = torch.Generator(device=DEV).manual_seed(3)
     d = torch.randn((p.n_traces, p.n_samples), generator=g, device=DEV)
@@ -64,9 +66,10 @@ def test_adjoint_dot_product_long_traces(n_samples):
     assert rel <= 1e-5
 
 
-@pytest.mark.parametrize("flags", [0, FLAG_AA])
+@pytest.mark.parametrize("flags", [0, FLAG_AA, FLAG_TAPER])
 def test_adjoint_matches_oracle(flags):
-    s, plan, p, vel = _setup(flags)
+    win = dict(min_offset_m=100.0, max_offset_m=500.0, taper_width=0.3) if flags & FLAG_TAPER else {}
+    s, plan, p, vel = _setup(flags, **win)
     g = torch.Generator(device=DEV).manual_seed(4)
     m = torch.randn(plan.image_shape, generator=g, device=DEV)
     out = plan.adjoint(m, vel.to(DEV))

@@ -113,14 +113,18 @@ def test_precision_3d_vt_velocity_slice():
 
 
 def test_compensated_accumulation_beats_plain_fp32():
-    # Summation error matters where an image sample adds many same-sign
-    # contributions: the precision generator's traces plus a constant (DC)
-    # level, migrated one offset class per call into one image (25 calls).
-    # Compensated (image + comp pair, TwoSum joins) against plain FP32
-    # accumulation, both against the FP64 oracle.
+    # Summation error dominates where an image sample adds many same-sign
+    # contributions of a smooth input: constant (DC) traces on the precision
+    # generator's geometry, migrated one offset class per call into one image
+    # (25 calls, up to 6400 contributions per sample).  Compensated (image +
+    # comp pair, TwoSum joins) against plain FP32 accumulation, both against
+    # the FP64 oracle, on the samples whose fold agrees with the oracle's (an
+    # aperture-edge flip changes a sample by a whole contribution) at 0.1 s
+    # and later (earlier, the oracle's 1e-6 m epsilons exceed 1e-8 of the
+    # weights).
     from seisproc_amd.processors.migration.hip_ops import PstmPlan
     s = S.precision_3d(seed=3, n_bins=16, n_classes=25, n_samples=401, n_img=16)
-    tr = s.traces + 1.0
+    tr = torch.ones_like(s.traces)
     ref, rfold = _oracle_forward(s, traces=tr)
     vel = s.vel.to(DEV)
     out = {}
@@ -135,9 +139,16 @@ def test_compensated_accumulation_beats_plain_fp32():
             pl.forward(tr[a:a + n_c].contiguous().to(DEV), vel, image=img, comp=comp, fold=fold)
             a += n_c
         torch.cuda.synchronize()
-        tot = img + comp if kahan else img
-        out[kahan] = _check(f"dc-{'kahan' if kahan else 'plain'}", tot, fold, ref, rfold, TOL_FP32)
-    print(f"[kahan-pin] compensated {out[True]:.3e} vs plain {out[False]:.3e}")
+        tot = (img.double() + comp.double()) if kahan else img.double()
+        g = tot.cpu().numpy()
+        f = fold.double().cpu().numpy()
+        tau = s.params.tau0 + s.params.dtau * np.arange(s.params.n_tau)
+        sel = (f == rfold) & (rfold > 0) & (tau >= 0.1)[None, None, :]
+        assert sel.sum() > 0.9 * ((rfold > 0) & (tau >= 0.1)[None, None, :]).sum()
+        out[kahan] = float(np.linalg.norm((g - ref)[sel]) / np.linalg.norm(ref[sel]))
+    print(f"[kahan-pin] rel_rms vs FP64 on {int(sel.sum())} samples: compensated {out[True]:.3e} "
+          f"vs plain {out[False]:.3e}", flush=True)
+    assert out[True] < TOL_FP32 and out[False] < TOL_FP32
     assert out[True] < 0.5 * out[False]
 
 

@@ -58,6 +58,13 @@ CASES = [
     ("utm-coordinates", dict(utm=True), dict()),
     ("kahan", dict(), dict(flags=FLAG_KAHAN)),
     ("obliquity", dict(lateral=True), dict(weight_mode=3)),
+    # the reference's taper composition with an offset window whose near and
+    # far tapers both act (offsets 150, 300, 450 m in [100, 500], taper 0.3)
+    ("taper-offset-window", dict(lateral=True), dict(flags=FLAG_TAPER | FLAG_KAHAN, min_offset_m=100.0,
+                                                     max_offset_m=500.0, taper_width=0.3)),
+    ("taper-aa-time-aperture", dict(), dict(flags=FLAG_TAPER | FLAG_AA | FLAG_TIME_APERTURE)),
+    # the window alone drops the 450 m class (no taper)
+    ("offset-window-only", dict(), dict(min_offset_m=100.0, max_offset_m=400.0)),
 ]
 
 

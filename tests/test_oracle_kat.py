@@ -95,6 +95,92 @@ def test_spreading_weight(oracle_lib):
     assert abs(w - 1.0 / (rs * rr + 1e-6)) < 1e-18
 
 
+# ---------------------------------------------------------------- taper composition
+def _taper(h_src, h_rcv, z, **kw):
+    """Weight of one pair under PSTM_FLAG_TAPER with the NONE weight (= the
+    taper product); 0 when the hard masks reject it.  Source and receiver on
+    opposite sides of the image point along x."""
+    v = 2000.0
+    p = _p(flags=FLAG_TAPER, weight_mode=WEIGHT_NONE, **kw)
+    ok, w, u, L = oracle.pair(p, h_src, 0.0, -h_rcv, 0.0, 2 * z / v, v)
+    return w if ok else 0.0
+
+
+def test_taper_distance_known_answers(oracle_lib):
+    # tests/test_kirchhoff_mvp.py:97-126 (ApertureController, aperture 1000 m,
+    # angle 90 deg, taper 0.1, z = 1000 m, source at the image point):
+    # 500, 800, 950 m pass, 1200, 2000 m are cut.  The cosine shape fixes the
+    # values: 1 up to 900 m, 0.5 at the middle of the taper zone (950 m),
+    # 0 at the limit (aperture.py:226-247).
+    kw = dict(aperture_m=1000.0, max_angle_deg=90.0, taper_width=0.1)
+    vals = [_taper(0.0, h, 1000.0, **kw) for h in (500.0, 800.0, 950.0, 1200.0, 2000.0)]
+    assert vals[0] == 1.0 and vals[1] == 1.0
+    assert abs(vals[2] - 0.5) < 1e-12
+    assert vals[3] == 0.0 and vals[4] == 0.0
+    assert _taper(0.0, 900.0, 1000.0, **kw) == 1.0
+    assert abs(_taper(0.0, 925.0, 1000.0, **kw) - 0.5 * (1 + math.cos(math.pi / 4))) < 1e-12
+    assert _taper(0.0, 999.999999, 1000.0, **kw) < 1e-12
+    # both legs multiply: 950 m on each side gives 0.25
+    assert abs(_taper(950.0, 950.0, 1000.0, **kw) - 0.25) < 1e-12
+
+
+def test_taper_angle_known_answers(oracle_lib):
+    # tests/test_kirchhoff_mvp.py:128-162 (aperture 10 km, angle 45 deg, taper
+    # 0.1, z = 1000 m): 100, 500, 800 m (38.7 deg, below the 40.5 deg taper
+    # start) pass with weight 1, 2000 m (63.4 deg) is cut.  In the taper zone
+    # the value is the cosine in angle (aperture.py:249-270), not in distance.
+    kw = dict(aperture_m=10000.0, max_angle_deg=45.0, taper_width=0.1)
+    vals = [_taper(h, 0.0, 1000.0, **kw) for h in (100.0, 500.0, 800.0, 2000.0)]
+    assert vals[:3] == [1.0, 1.0, 1.0] and vals[3] == 0.0
+    th0, th1 = math.radians(40.5), math.radians(45.0)
+    for deg in (41.0, 42.75, 44.0):
+        h = 1000.0 * math.tan(math.radians(deg))
+        expect = 0.5 * (1 + math.cos(math.pi * (math.radians(deg) - th0) / (th1 - th0)))
+        got = _taper(h, 0.0, 1000.0, **kw)
+        assert abs(got - expect) < 1e-9, (deg, got, expect)
+        # a taper in tan-angle (distance) space differs by far more
+        x = (h - 1000.0 * math.tan(th1) * 0.9) / (1000.0 * math.tan(th1) * 0.1)
+        assert abs(got - 0.5 * (1 + math.cos(math.pi * min(max(x, 0), 1)))) > 1e-3
+    assert abs(_taper(1000.0 * math.tan(math.radians(42.75)), 0.0, 1000.0, **kw) - 0.5) < 1e-12
+
+
+@pytest.mark.parametrize("off,expect", [
+    (50.0, 0.0), (100.0, 0.0), (150.0, 0.5), (200.0, 1.0), (600.0, 1.0),
+    (950.0, 0.5), (1000.0, 0.0), (1050.0, 0.0),
+])
+def test_offset_window_known_answers(oracle_lib, off, expect):
+    # aperture.py:184-224 with min 100 m, max 1000 m, taper 0.1 (taper range
+    # 90 m): 0 outside [min, max]; near taper 0.5 (1 - cos(pi (o - min)/90)) on
+    # [100, 190); far taper 0.5 (1 + cos(pi (o - 910)/90)) on (910, 1000]
+    off_kw = dict(min_offset_m=100.0, max_offset_m=1000.0, taper_width=0.1)
+    if off == 150.0:
+        expect = 0.5 * (1 - math.cos(math.pi * 50.0 / 90.0))
+    if off == 950.0:
+        expect = 0.5 * (1 + math.cos(math.pi * 40.0 / 90.0))
+    # a zero-offset-free pair at a deep image point: only the offset window acts
+    got = _taper(0.5 * off, 0.5 * off, 3000.0, aperture_m=1e5, max_angle_deg=89.0, **off_kw)
+    assert abs(got - expect) < 1e-12, (off, got, expect)
+
+
+def test_offset_window_drops_traces_without_taper(oracle_lib):
+    # outside [min, max] a trace contributes nothing and counts no fold, with
+    # or without the taper flag; inside, without the flag, nothing is weighted
+    s = S.tiny_survey(seed=8, n_bins=6, n_classes=3, n_samples=121, n_img=3, aperture=300.0)
+    sx, sy, rx, ry = s.numpy_geometry()
+    d = s.traces.double().numpy()
+    v = s.vel.double().numpy()
+    off = np.hypot(rx - sx, ry - sy)
+    lo, hi = 200.0, 400.0  # keeps the 300 m class of 150/300/450 m
+    keep = (off >= lo) & (off <= hi)
+    assert 0 < keep.sum() < len(off)
+    p = s.params.replace(vel_mode=0)
+    full, ffold = oracle.forward(p.replace(min_offset_m=lo, max_offset_m=hi), d, sx, sy, rx, ry, v)
+    sub, sfold = oracle.forward(p.replace(n_traces=int(keep.sum())), d[keep], sx[keep], sy[keep], rx[keep],
+                                ry[keep], v)
+    np.testing.assert_array_equal(ffold, sfold)
+    np.testing.assert_allclose(full, sub, rtol=0, atol=1e-14 * np.abs(sub).max())
+
+
 # ---------------------------------------------------------------- interpolation
 def _column_trace(samples, tau_list, t0=0.0, dt=0.002, weight=WEIGHT_NONE, flags=0, aa_max=16.0):
     """Zero-offset trace at the single image column: t = tau exactly."""
@@ -275,12 +361,16 @@ def test_vti_forward_equals_isotropic_at_eta_zero(oracle_lib):
 @pytest.mark.parametrize("flags,lateral,wm", [
     (0, False, 0), (FLAG_AA, False, 0), (FLAG_AA | FLAG_VTI, True, 0), (FLAG_TAPER, True, 1),
     (FLAG_TIME_APERTURE | FLAG_ZERO_OUTSIDE, False, 2), (0, True, 3),
+    (FLAG_TAPER | FLAG_TIME_APERTURE | FLAG_AA, False, 0),
 ])
 def test_oracle_matches_numpy_brute_force(oracle_lib, flags, lateral, wm):
     s = S.tiny_survey(seed=4, n_bins=5, n_classes=2, n_samples=121, n_img=3, aperture=250.0, flags=flags,
                       lateral=lateral)
     sx, sy, rx, ry = s.numpy_geometry()
     p = s.params.replace(weight_mode=wm, vel_mode=1 if s.vel.dim() == 3 else 0)
+    if flags & FLAG_TAPER:
+        # offsets 150 and 300 m: the window's near taper acts on both classes
+        p = p.replace(min_offset_m=100.0, max_offset_m=700.0, taper_width=0.4)
     d = s.traces.double().numpy()
     v = s.vel.double().numpy()
     eta = s.eta.double().numpy() if s.eta is not None else None
@@ -305,6 +395,8 @@ def test_oracle_dot_product(oracle_lib, flags):
                       flags=flags & (FLAG_AA | FLAG_VTI), lateral=True)
     sx, sy, rx, ry = s.numpy_geometry()
     p = s.params.replace(flags=flags, vel_mode=1)
+    if flags & FLAG_TAPER:
+        p = p.replace(min_offset_m=100.0, max_offset_m=700.0)
     v = s.vel.double().numpy()
     if v.ndim == 1:
         v = np.broadcast_to(v, (p.n_il, p.n_xl, p.n_tau)).copy()
