@@ -115,8 +115,11 @@ def test_precision_3d_vt_velocity_slice():
 def test_compensated_accumulation_beats_plain_fp32():
     # Summation error dominates where an image sample adds many same-sign
     # contributions of a smooth input: constant (DC) traces on the precision
-    # generator's geometry, migrated one offset class per call into one image
-    # (25 calls, up to 6400 contributions per sample).  Compensated (image +
+    # generator's geometry, migrated a quarter of an offset class per call
+    # into one image (100 calls, up to 6400 contributions per sample; plain
+    # FP32 accumulation error grows with the calls, the compensated pair's
+    # does not: what remains is the FP32 arithmetic of each contribution).
+    # Compensated (image +
     # comp pair, TwoSum joins) against plain FP32 accumulation, both against
     # the FP64 oracle, on the samples whose fold agrees with the oracle's (an
     # aperture-edge flip changes a sample by a whole contribution) at 0.1 s
@@ -132,12 +135,12 @@ def test_compensated_accumulation_beats_plain_fp32():
         img = torch.zeros(s.params.n_il, s.params.n_xl, s.params.n_tau, device=DEV)
         comp = torch.zeros_like(img) if kahan else None
         fold = torch.zeros_like(img)
-        a = 0
-        for n_c in s.class_sizes:
-            p = s.params.replace(n_traces=n_c, flags=FLAG_KAHAN if kahan else 0)
-            pl = PstmPlan(p, s.src_xy[a:a + n_c], s.rcv_xy[a:a + n_c])
-            pl.forward(tr[a:a + n_c].contiguous().to(DEV), vel, image=img, comp=comp, fold=fold)
-            a += n_c
+        bounds = np.cumsum([0] + [q for n_c in s.class_sizes for q in (n_c // 4,) * 3 + (n_c - 3 * (n_c // 4),)])
+        assert len(bounds) == 101 and bounds[-1] == s.n_traces
+        for a, b in zip(bounds[:-1], bounds[1:]):
+            p = s.params.replace(n_traces=int(b - a), flags=FLAG_KAHAN if kahan else 0)
+            pl = PstmPlan(p, s.src_xy[a:b], s.rcv_xy[a:b])
+            pl.forward(tr[a:b].contiguous().to(DEV), vel, image=img, comp=comp, fold=fold)
         torch.cuda.synchronize()
         tot = (img.double() + comp.double()) if kahan else img.double()
         g = tot.cpu().numpy()
