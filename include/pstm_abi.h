@@ -173,6 +173,24 @@ int pstm_vel_grad(const pstm_params_t* p, const void* ws, const void* traces,
                   const float* trace_scale, const float* vel, const float* image_grad,
                   float* vel_grad, void* stream);
 
+/* Size of the static plan workspace of pstm_migrate_static (per GPU). */
+#define PSTM_STATIC_SCRATCH_BYTES ((size_t)32 << 20)
+
+/* pstm_plan_build followed by pstm_forward with the plan kept in a static
+ * device array of PSTM_STATIC_SCRATCH_BYTES instead of a caller workspace:
+ * the call allocates no memory (no hipMalloc, hipFree or allocating library
+ * routine), for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation).  No counterpart in
+ * the reference.  Arguments as pstm_plan_build and pstm_forward; returns
+ * PSTM_ERR_WORKSPACE when the plan of p->n_traces traces needs more than the
+ * static workspace (about 56 bytes per trace plus the sort and cell tables:
+ * some 570,000 traces for a 400 x 400-column image with a 1500 m aperture;
+ * split larger inputs over several calls, which accumulate).  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams or threads) race. */
+int pstm_migrate_static(const pstm_params_t* p, const double* src_xy, const double* rcv_xy,
+                        const void* traces, const float* trace_scale, const float* vel, const float* eta,
+                        float* image, float* image_comp, float* fold, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

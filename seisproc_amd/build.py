@@ -18,7 +18,7 @@ CSRC = HERE / "csrc"
 OUT_DIR = HERE / "_obj"
 LIB = HERE / "libpstm_hip.so"
 SOURCES = ["pstm_abi.hip", "pstm_plan.hip", "pstm_fwd_f32.hip", "pstm_fwd_bf16.hip", "pstm_fwd_f16.hip",
-           "pstm_adj.hip", "pstm_grad.hip"]
+           "pstm_adj.hip", "pstm_grad.hip", "pstm_static.hip"]
 ARCH = os.environ.get("PSTM_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

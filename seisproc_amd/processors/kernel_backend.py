@@ -34,6 +34,7 @@ SYMBOLS = {
                                           _V]),
     "pstm_adjoint": (ctypes.c_int, [_P, _V, _V, _V, _V, _V, _V]),
     "pstm_vel_grad": (ctypes.c_int, [_P, _V, _V, _V, _V, _V, _V, _V]),
+    "pstm_migrate_static": (ctypes.c_int, [_P, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
 }
 
 
