@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import logging
 import time
+from dataclasses import dataclass
 from typing import Any, Callable, Dict, List, Optional, Tuple
 
 import numpy as np
@@ -123,3 +124,103 @@ class KirchhoffMigrator(BaseMigrator, BaseProcessor):
 
 def create_kirchhoff_migrator(velocity, config: MigrationConfig, device=None, **kw) -> KirchhoffMigrator:
     return KirchhoffMigrator(velocity, config, device=device, **kw)
+
+
+@dataclass
+class OptimizationStats:
+    """Statistics of the last migrate_gather call, with the reference's fields
+    (processors/migration/optimized_kirchhoff_migrator.py:35-48).  This
+    backend fills the trace counts and time_accumulation (the whole call);
+    it has no traveltime table or reciprocity pairs, so those stay 0."""
+
+    total_traces: int = 0
+    traces_processed: int = 0
+    trace_reduction_ratio: float = 1.0
+    depth_groups_used: int = 0
+    reciprocal_pairs: int = 0
+    traveltime_lut_hits: int = 0
+    memory_peak_mb: float = 0.0
+    time_traveltime: float = 0.0
+    time_aperture: float = 0.0
+    time_interpolation: float = 0.0
+    time_accumulation: float = 0.0
+
+
+class OptimizedKirchhoffMigrator(KirchhoffMigrator):
+    """The interactive path's migrator (reference
+    processors/migration/optimized_kirchhoff_migrator.py:50-719, called from
+    main_window.py:4642-4653) on the HIP backend.
+
+    Same constructor flags.  They name torch-side strategies of the
+    reference that the HIP kernel carries out by construction, so they change
+    nothing here and are kept for the interface and its description:
+    ``enable_spatial_index`` and ``enable_adaptive_aperture`` are the plan's
+    midpoint cells and each tile's aperture/angle reach, ``enable_gpu_tiling``
+    the tile kernel itself, ``enable_lut`` is not needed (the DSR time costs
+    two reciprocal square roots), and ``enable_symmetry`` is off in the
+    reference too.  The sum is the reference tile loop's
+    (:356-562): SPREADING weight 1/(r_s r_r) (the configuration's weight
+    mode), hard aperture and angle masks, linear interpolation.
+    """
+
+    def __init__(self, velocity, config, device: Optional[torch.device] = None, enable_lut: bool = True,
+                 enable_adaptive_aperture: bool = True, enable_spatial_index: bool = True,
+                 enable_symmetry: bool = False, enable_gpu_tiling: bool = True, **kw):
+        self._opts = dict(lut=bool(enable_lut), adaptive_aperture=bool(enable_adaptive_aperture),
+                          spatial_index=bool(enable_spatial_index), symmetry=bool(enable_symmetry),
+                          gpu_tiling=bool(enable_gpu_tiling))
+        self._stats = OptimizationStats()
+        self._profile_tiles = False
+        super().__init__(velocity, config, device=device, **kw)
+
+    def migrate_gather(self, gather: SeismicData, geometry: MigrationGeometry,
+                       output_image: Optional[torch.Tensor] = None,
+                       output_fold: Optional[torch.Tensor] = None) -> Tuple[torch.Tensor, torch.Tensor]:
+        t0 = time.time()
+        image, fold = super().migrate_gather(gather, geometry, output_image, output_fold)
+        torch.cuda.synchronize(self.device)
+        off = geometry.offset
+        used = int(np.count_nonzero((off >= self.config.min_offset_m) & (off <= self.config.max_offset_m)))
+        n = int(gather.n_traces)
+        self._stats = OptimizationStats(total_traces=n, traces_processed=used,
+                                        trace_reduction_ratio=used / n if n else 1.0,
+                                        memory_peak_mb=torch.cuda.max_memory_allocated(self.device) / 2**20,
+                                        time_accumulation=time.time() - t0)
+        return image, fold
+
+    def get_optimization_stats(self) -> OptimizationStats:
+        """Statistics of the last migrate_gather call."""
+        return self._stats
+
+    def enable_profiling(self, enable: bool = True) -> None:
+        """Kept for the interface: per-tile timing is taken with rocprofv3."""
+        self._profile_tiles = bool(enable)
+
+    def get_description(self) -> str:
+        o = self._opts
+        return (super().get_description() +
+                f"\n  Optimizations: LUT={o['lut']}, AdaptiveAperture={o['adaptive_aperture']}, "
+                f"SpatialIndex={o['spatial_index']}, Symmetry={o['symmetry']}, GPUTiling={o['gpu_tiling']}")
+
+
+_OPT_LEVELS = {
+    "low": dict(enable_lut=True, enable_adaptive_aperture=False, enable_spatial_index=False, enable_symmetry=False,
+                enable_gpu_tiling=False),
+    "medium": dict(enable_lut=True, enable_adaptive_aperture=True, enable_spatial_index=True, enable_symmetry=False,
+                   enable_gpu_tiling=False),
+    "high": dict(enable_lut=True, enable_adaptive_aperture=True, enable_spatial_index=True, enable_symmetry=False,
+                 enable_gpu_tiling=True),
+}
+
+
+def create_optimized_kirchhoff_migrator(velocity, config: MigrationConfig, prefer_gpu: bool = True,
+                                        optimization_level: str = "high") -> OptimizedKirchhoffMigrator:
+    """Factory of the interactive path (reference
+    optimized_kirchhoff_migrator.py:721-775): same keywords and levels
+    ('low', 'medium', anything else = 'high').  The migration runs on the
+    GPU only: ``prefer_gpu=False`` is logged and the GPU is used; without a
+    ROCm GPU the constructor raises (there is no CPU path)."""
+    if not prefer_gpu:
+        logger.warning("prefer_gpu=False: the Kirchhoff migration runs on the GPU only (HIP backend)")
+    opts = _OPT_LEVELS.get(optimization_level, _OPT_LEVELS["high"])
+    return OptimizedKirchhoffMigrator(velocity, config, **opts)

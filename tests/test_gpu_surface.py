@@ -145,6 +145,36 @@ def test_kirchhoff_migrator_dataset_equals_single_gather():
     assert np.linalg.norm(one.image - refn) / np.linalg.norm(refn) < 1e-4
 
 
+def test_optimized_migrator_factory_equals_class_api():
+    # the interactive path's factory (main_window.py:4642-4653 calls it with
+    # prefer_gpu and optimization_level='high') runs the same HIP sum as
+    # KirchhoffMigrator: bit-identical image and fold at every level
+    from seisproc_amd.processors.migration import KirchhoffMigrator, create_optimized_kirchhoff_migrator
+    s = _zero_offset_survey()
+    p = s.params
+    g = OutputGrid(n_time=p.n_tau, n_inline=p.n_il, n_xline=p.n_xl, dt=p.dtau, d_inline=p.il_dx, d_xline=p.xl_dy,
+                   x_origin=p.origin_x, y_origin=p.origin_y)
+    cfg = MigrationConfig(output_grid=g, max_aperture_m=p.aperture_m, antialias_enabled=False)
+    sx, sy, rx, ry = s.numpy_geometry()
+    gather = SeismicData(s.traces.t().numpy().copy(), p.dt * 1000.0)
+    geo = MigrationGeometry(sx, sy, rx, ry)
+    ref_img, ref_fold = KirchhoffMigrator(VelocityModel(2000.0), cfg).migrate_gather(gather, geo)
+    for level in ("low", "medium", "high"):
+        mig = create_optimized_kirchhoff_migrator(velocity=VelocityModel(2000.0), config=cfg, prefer_gpu=True,
+                                                  optimization_level=level)
+        img, fold = mig.migrate_gather(gather=gather, geometry=geo)
+        assert torch.equal(img, ref_img) and torch.equal(fold, ref_fold)
+        st = mig.get_optimization_stats()
+        assert st.total_traces == s.n_traces and st.traces_processed == s.n_traces
+        assert "GPUTiling=" + str(level == "high") in mig.get_description()
+    # SPREADING weight against the oracle (raw image and fold, [tau][il][xl])
+    ref, rfold = oracle.forward(p.replace(weight_mode=1), s.traces.double().numpy(), sx, sy, rx, ry,
+                                np.full(p.n_tau, 2000.0))
+    np.testing.assert_array_equal(ref_fold.cpu().numpy(), rfold.transpose(2, 0, 1))
+    r = ref.transpose(2, 0, 1)
+    assert np.linalg.norm(ref_img.cpu().numpy() - r) / np.linalg.norm(r) < 1e-4
+
+
 def test_processor_vti_with_anisotropy_model():
     from seisproc_amd.processors import create_processor
     s = _zero_offset_survey()

@@ -221,3 +221,17 @@ def test_pstm_processor_params_roundtrip_without_gpu():
         PROCESSOR_REGISTRY["PSTMProcessor"](config=cfg, velocity=2000.0, trace_dtype="int8")
     with pytest.raises(ValueError):
         BaseProcessor.from_dict({"module": "seisproc_amd.models", "class_name": "SeismicData", "params": {}})
+
+
+def test_optimized_migrator_factory_has_no_cpu_path():
+    # create_optimized_kirchhoff_migrator (optimized_kirchhoff_migrator.py:721-775)
+    # keeps the reference's keywords; without a ROCm GPU it raises instead of
+    # falling back to a CPU migration
+    import torch
+    from seisproc_amd.models import MigrationConfig, VelocityModel
+    from seisproc_amd.processors.migration import create_optimized_kirchhoff_migrator
+    if torch.cuda.is_available():
+        pytest.skip("checks the no-GPU behaviour")
+    with pytest.raises(RuntimeError, match="GPU"):
+        create_optimized_kirchhoff_migrator(velocity=VelocityModel(2000.0), config=MigrationConfig(),
+                                            prefer_gpu=False, optimization_level="medium")
