@@ -75,12 +75,12 @@ static int validate(const pstm_params_t* p) {
     if (!(p->aa_scale >= 0)) return set_error(PSTM_ERR_INVALID, "aa_scale must be >= 0");
     pad = (int)ceil(p->aa_max) + 2;
   }
-  // A staged window never exceeds n_samples + 2 (pad + margin) + 1 samples.
-  const bool aa = (p->flags & PSTM_FLAG_AA) != 0;
-  const int64_t cap = min_cap_elems(aa);
-  if (whole_window(p, pad) > cap)
-    return set_error(PSTM_ERR_UNSUPPORTED, "n_samples %d too long for the LDS sample buffer (%lld samples%s)",
-                     p->n_samples, (long long)cap, aa ? " with antialias" : "");
+  // Any record length: a trace window longer than a kernel form's LDS share
+  // is staged in segments.  The bound keeps window and sample indices far
+  // from the 32-bit range.
+  (void)pad;
+  if (p->n_samples > (1 << 24))
+    return set_error(PSTM_ERR_UNSUPPORTED, "n_samples %d above 16,777,216", p->n_samples);
   if ((int64_t)p->n_il * p->n_xl * (int64_t)p->n_tau > ((int64_t)1 << 40))
     return set_error(PSTM_ERR_INVALID, "image too large");
   return PSTM_OK;

@@ -535,28 +535,16 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
     auto process_queue = [&](bool final_flush) {
       int qs = 0;
       const int qn = S.queue_n;
-      while (true) {
-        const int rem = qn - qs;
-        if (rem <= 0 || (!final_flush && rem < MB)) break;
-        if (wave == 0) {
-          const int i = qs + lane;
-          const bool in = lane < MB && i < qn;
-          const int cost = in ? (SKEW ? skew_len(S.q_len[i]) : S.q_len[i]) : 0;  // window elements
-          const int incl = wave_scan(cost, lane);
-          const bool fit = in && incl <= CAPE;
-          const uint64_t m = __ballot(fit);
-          if (fit) {
-            const int off = incl - cost;
-            const int w0 = S.q_w0[i];
-            S.b_off[lane] = off;
-            // SKEW: sample i at base + i + (i >> 5), one spare slot before the window (w0 >= 0)
-            S.b_base[lane] = SKEW ? off + 1 - (w0 + (w0 >> 5)) : off - w0;
-          }
-          if (lane == 0) S.nb = __popcll(m);
-        }
-        __syncthreads();
-        const int nb = S.nb;
-
+      // One sub-batch: the nb queued windows from queue position qs, placed
+      // at S.b_off / S.b_base.  SEG: a single window that is one segment of a
+      // longer one; only the contributions whose sample index lies in
+      // [sg_lo, sg_hi) are summed (every contribution of the trace belongs to
+      // exactly one segment), and every lane's read position is held inside
+      // the staged segment.
+      auto run_batch = [&](const int qs, const int nb, auto seg_tag, const int sg_lo, const int sg_hi) {
+        constexpr bool SEG = decltype(seg_tag)::value;
+        (void)sg_lo;
+        (void)sg_hi;
         // -- stage (FWD/GRAD: load samples; ADJ: zero the windows)
         for (int b = wave; b < nb; b += kWaves) {
           const int e = qs + b;
@@ -735,15 +723,26 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               L = __builtin_amdgcn_fmed3f(qa[c] * (fabsf(ga) + fabsf(gb)), 1.f, a.aa_max);
               iL2 = __builtin_amdgcn_rcpf(L * L);
             }
-            const int i0 = (int)u;  // u >= 0: truncation is floor
-            const float f = __builtin_amdgcn_fractf(u);
+            // SEG: only sample indices of this segment count; the read
+            // position of the other lanes is held inside the staged segment
+            [[maybe_unused]] float us = u;
+            if constexpr (SEG) {
+              static_assert(KIND == 0, "segments take the general loop");
+              const int it = (int)u;
+              const bool ins = it >= sg_lo && it < sg_hi;
+              ok = ok && ins;
+              mk = ins ? mk : 0.f;
+              us = __builtin_amdgcn_fmed3f(u, (float)sg_lo, (float)sg_hi - 0.5f);
+            }
+            const int i0 = (int)us;  // us >= 0: truncation is floor
+            const float f = __builtin_amdgcn_fractf(us);
             if constexpr (MODE == MODE_FWD || MODE == MODE_GRAD) {
               float amp;
               [[maybe_unused]] float a0 = 0.f, a1 = 0.f;
               if constexpr (AA) {
                 // triangle taps: second difference of the double integral A,
                 // A(i + f) = A(i) - f C(i) with C the single integral
-                const float um = u - L, up = u + L;
+                const float um = us - L, up = us + L;
                 const float flm = floorf(um);
                 const int im = (int)flm, ip = (int)up;
                 const float fm = um - flm, fp = __builtin_amdgcn_fractf(up);
@@ -808,7 +807,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
                 } else if constexpr (FX) {
                   const double g = (double)gsc * fx_scale, gf = g * (double)f;
                   if constexpr (AA) {
-                    const float um = u - L, up = u + L;
+                    const float um = us - L, up = us + L;
                     const int im = (int)floorf(um), ip = (int)up;
                     const double fm = (double)(um - (float)im), fp = (double)(up - (float)ip);
                     const double gc = 2.0 * g * (double)iL2, ge = -g * (double)iL2, gcf = gc * (double)f;
@@ -823,7 +822,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
                     atomicAdd(&Lb[i0 + 1], to_fixed64(gf));
                   }
                 } else if constexpr (AA) {
-                  const float um = u - L, up = u + L;
+                  const float um = us - L, up = us + L;
                   const int im = (int)floorf(um), ip = (int)up;
                   const float fm = um - (float)im, fp = up - (float)ip;
                   const double gc = 2.0 * (double)(gsc * iL2), ge = -(double)(gsc * iL2);
@@ -913,7 +912,10 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
         };
         const uint64_t todo = __ballot(lane < nb && ((P.act[lane] >> wave) & 1));
         uint64_t tfull = 0;
-        if constexpr (FULLPATH) {
+        if constexpr (SEG) {
+          if (!SKEW || narrow) walk(todo, std::integral_constant<int, 0>{}, std::true_type{});
+          else walk(todo, std::integral_constant<int, 0>{}, std::false_type{});
+        } else if constexpr (FULLPATH) {
           tfull = __ballot(lane < nb && ((P.act[lane] >> (4 + wave)) & 1));
           const uint64_t tunm = __ballot(lane < nb && ((P.act[lane] >> (8 + wave)) & 1));
           nfull += (int)__popcll(tfull) + (int)__popcll(tunm);
@@ -921,7 +923,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           walk(tunm, std::integral_constant<int, 2>{}, std::true_type{});
           tfull |= tunm;
         }
-        if constexpr (MODE == MODE_ADJ && !AA) {
+        if constexpr (SEG) {
+        } else if constexpr (MODE == MODE_ADJ && !AA) {
           // traces inside the record in this wave, then those that clamp
           tfull = __ballot(lane < nb && ((P.act[lane] >> (4 + wave)) & 1));
           if (!SKEW || narrow) {
@@ -979,7 +982,60 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           }
           __syncthreads();
         }
-        qs += nb;
+      };
+      while (true) {
+        const int rem = qn - qs;
+        if (rem <= 0 || (!final_flush && rem < MB)) break;
+        if (wave == 0) {
+          const int i = qs + lane;
+          const bool in = lane < MB && i < qn;
+          const int cost = in ? (SKEW ? skew_len(S.q_len[i]) : S.q_len[i]) : 0;  // window elements
+          const int incl = wave_scan(cost, lane);
+          const bool fit = in && incl <= CAPE;
+          const uint64_t m = __ballot(fit);
+          if (fit) {
+            const int off = incl - cost;
+            const int w0 = S.q_w0[i];
+            S.b_off[lane] = off;
+            // SKEW: sample i at base + i + (i >> 5), one spare slot before the window (w0 >= 0)
+            S.b_base[lane] = SKEW ? off + 1 - (w0 + (w0 >> 5)) : off - w0;
+          }
+          if (lane == 0) S.nb = __popcll(m);
+        }
+        __syncthreads();
+        const int nb = S.nb;
+        if (nb > 0) {
+          run_batch(qs, nb, std::false_type{}, 0, 0);
+          qs += nb;
+          continue;
+        }
+        // The first window alone exceeds the LDS share: it is processed in
+        // segments of at most seg_cap staged elements, each assigned the
+        // sample indices [lo, hi) and staged with the taps' reach on either
+        // side (the next sample; with antialias the triangle's half-width
+        // and its neighbour), clipped to the window.
+        {
+          const int w0 = S.q_w0[qs], len = S.q_len[qs];
+          constexpr int seg_cap = SKEW ? ((CAPE - 2) * 32) / 33 : CAPE;
+          // room for a segment beyond the widest taps' reach (antialias pad <= 258)
+          static_assert(seg_cap > 2 * 259 + 1 + 64, "LDS share too small for segments");
+          const int pad_lo = AA ? a.aa_pad + 1 : 0, pad_hi = AA ? a.aa_pad + 2 : 1;
+          const int step = seg_cap - pad_lo - pad_hi;
+          for (int lo = w0; lo < w0 + len; lo += step) {
+            const int hi = min(lo + step, w0 + len);
+            const int s0 = max(lo - pad_lo, w0), s1 = min(hi + pad_hi, w0 + len);
+            __syncthreads();  // every thread has read the window before it is rewritten
+            if (tid == 0) {
+              S.q_w0[qs] = s0;
+              S.q_len[qs] = s1 - s0;
+              S.b_off[0] = 0;
+              S.b_base[0] = SKEW ? 1 - (s0 + (s0 >> 5)) : -s0;
+            }
+            __syncthreads();
+            run_batch(qs, 1, std::true_type{}, lo, hi);
+          }
+        }
+        qs += 1;
       }
       // move the unprocessed tail (< MB entries) to the front
       const int rem = qn - qs;

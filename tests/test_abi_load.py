@@ -52,11 +52,14 @@ def _p(**kw):
     (dict(dt=0.0), PSTM_ERR_INVALID, "dt"),
     (dict(max_angle_deg=95.0), PSTM_ERR_INVALID, "max_angle"),
     (dict(xl_dx=10.0, xl_dy=10.0), PSTM_ERR_INVALID, "orthogonal"),
-    (dict(n_samples=10600), PSTM_ERR_UNSUPPORTED, "n_samples"),
-    (dict(n_samples=10500), PSTM_OK, ""),
+    # any record length (windows longer than the LDS share are staged in segments)
+    (dict(n_samples=10600), PSTM_OK, ""),
+    (dict(n_samples=120000), PSTM_OK, ""),
+    (dict(n_samples=(1 << 24) + 1), PSTM_ERR_UNSUPPORTED, "n_samples"),
+    (dict(n_samples=1), PSTM_ERR_INVALID, "n_samples"),
     (dict(flags=FLAG_AA, aa_max=0.5), PSTM_ERR_INVALID, "aa_max"),
-    (dict(flags=FLAG_AA, n_samples=5100), PSTM_ERR_UNSUPPORTED, "with antialias"),
-    (dict(flags=FLAG_AA, n_samples=5000), PSTM_OK, ""),
+    (dict(flags=FLAG_AA, n_samples=50000), PSTM_OK, ""),
+    (dict(min_offset_m=500.0, max_offset_m=400.0), PSTM_ERR_INVALID, "min_offset_m"),
     (dict(flags=1 << 12), PSTM_ERR_INVALID, "flag"),
     (dict(tau0=-0.1), PSTM_ERR_INVALID, "tau0"),
 ])

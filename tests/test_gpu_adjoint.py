@@ -66,6 +66,32 @@ def test_adjoint_dot_product_long_traces(n_samples):
     assert rel <= 1e-5
 
 
+@pytest.mark.parametrize("n_samples,dtau,flags", [(12001, 0.024, 0), (8001, 0.016, FLAG_AA)])
+def test_adjoint_segmented_long_windows(n_samples, dtau, flags):
+    # windows longer than the LDS share (one 256-time block spans the record):
+    # the adjoint stages them in segments too; dot-product test and the
+    # oracle's transpose
+    from seisproc_amd.processors.migration.hip_ops import PstmPlan
+    s = S.tiny_survey(seed=24, n_bins=6, n_classes=2, n_samples=n_samples, n_img=4, aperture=2500.0)
+    n_tau = int(round((n_samples - 1) * 0.0005 / dtau))
+    p = s.params.replace(dt=0.0005, n_tau=n_tau, dtau=dtau, flags=flags, max_angle_deg=85.0, vel_mode=0)
+    vel = torch.full((n_tau,), 1500.0, device=DEV)
+    plan = PstmPlan(p, s.src_xy, s.rcv_xy)
+    g = torch.Generator(device=DEV).manual_seed(8)
+    d = torch.randn((p.n_traces, p.n_samples), generator=g, device=DEV)
+    m = torch.randn(plan.image_shape, generator=g, device=DEV)
+    Ld, _ = plan.forward(d, vel, with_fold=False)
+    Ltm = plan.adjoint(m, vel)
+    lhs = float((Ld.double() * m.double()).sum())
+    rhs = float((d.double() * Ltm.double()).sum())
+    rel = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    sx, sy, rx, ry = s.numpy_geometry()
+    ref = oracle.adjoint(p, m.double().cpu().numpy(), sx, sy, rx, ry, vel.double().cpu().numpy())
+    err = np.linalg.norm(Ltm.double().cpu().numpy() - ref) / np.linalg.norm(ref)
+    print(f"[segmented adjoint n={n_samples} flags={flags}] dot rel={rel:.2e} oracle rel_rms={err:.3e}")
+    assert rel <= 1e-5 and err <= 1e-4
+
+
 @pytest.mark.parametrize("flags", [0, FLAG_AA, FLAG_TAPER])
 def test_adjoint_matches_oracle(flags):
     win = dict(min_offset_m=100.0, max_offset_m=500.0, taper_width=0.3) if flags & FLAG_TAPER else {}

@@ -186,6 +186,30 @@ def test_empty_and_unreachable_traces():
     assert pl.kept() == 0 and float(fold.sum()) == 0
 
 
+@pytest.mark.parametrize("n_samples,dtau,flags", [
+    # windows longer than every form's LDS share, staged in segments: output
+    # times far coarser than the input sampling make one 256-time block span
+    # the whole record (12,001 samples plain, 8,001 with antialias)
+    (12001, 0.024, 0), (8001, 0.016, FLAG_AA), (8001, 0.016, FLAG_AA | FLAG_VTI), (12001, 0.024, FLAG_TAPER),
+])
+def test_segmented_long_windows(n_samples, dtau, flags):
+    from seisproc_amd.processors.migration.hip_ops import PstmPlan
+    s = S.tiny_survey(seed=19, n_bins=6, n_classes=2, n_samples=n_samples, n_img=4, aperture=2500.0,
+                      flags=flags & (FLAG_AA | FLAG_VTI))
+    # input at 0.5 ms; 250 output times at dtau (the record, 6 s / 4 s, in one block)
+    n_tau = int(round((n_samples - 1) * 0.0005 / dtau))
+    p = s.params.replace(dt=0.0005, n_tau=n_tau, dtau=dtau, flags=flags, max_angle_deg=85.0,
+                         vel_mode=1 if s.vel.dim() == 3 else 0)
+    vel = torch.full((n_tau,), 1500.0) if s.vel.dim() == 1 else torch.full((4, 4, n_tau), 1500.0)
+    eta = None if s.eta is None else s.eta[:, :, :n_tau].contiguous()
+    plan = PstmPlan(p, s.src_xy, s.rcv_xy)
+    img, fold = plan.forward(s.traces.to(DEV), vel.to(DEV), eta=None if eta is None else eta.to(DEV))
+    sx, sy, rx, ry = s.numpy_geometry()
+    ref, rfold = oracle.forward(p, s.traces.double().numpy(), sx, sy, rx, ry, vel.double().numpy(),
+                                eta=None if eta is None else eta.double().numpy())
+    _compare(f"segmented-{n_samples}-flags{flags}", img, fold, ref, rfold)
+
+
 @pytest.mark.parametrize("n_samples,flags", [(10500, 0), (6000, 0), (4027, 0), (4026, 0), (3000, 0), (2523, 0),
                                              (2522, 0), (2075, 0), (2074, 0), (5000, FLAG_AA), (1900, FLAG_AA)])
 def test_long_traces_near_lds_limit(n_samples, flags):
@@ -215,4 +239,4 @@ def test_input_validation_errors():
     with pytest.raises(ValueError):
         plan.forward(s.traces.to(DEV)[:, :50], s.vel.to(DEV))  # wrong shape
     with pytest.raises(RuntimeError, match="n_samples"):
-        PstmPlan(s.params.replace(n_samples=100000), s.src_xy, s.rcv_xy)
+        PstmPlan(s.params.replace(n_samples=1), s.src_xy, s.rcv_xy)
