@@ -541,10 +541,12 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
       // [sg_lo, sg_hi) are summed (every contribution of the trace belongs to
       // exactly one segment), and every lane's read position is held inside
       // the staged segment.
-      auto run_batch = [&](const int qs, const int nb, auto seg_tag, const int sg_lo, const int sg_hi) {
+      auto run_batch = [&](const int qs, const int nb, auto seg_tag, const int sg_lo, const int sg_hi,
+                           const float sg_rd) {
         constexpr bool SEG = decltype(seg_tag)::value;
         (void)sg_lo;
         (void)sg_hi;
+        (void)sg_rd;
         // -- stage (FWD/GRAD: load samples; ADJ: zero the windows)
         for (int b = wave; b < nb; b += kWaves) {
           const int e = qs + b;
@@ -723,8 +725,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               L = __builtin_amdgcn_fmed3f(qa[c] * (fabsf(ga) + fabsf(gb)), 1.f, a.aa_max);
               iL2 = __builtin_amdgcn_rcpf(L * L);
             }
-            // SEG: only sample indices of this segment count; the read
-            // position of the other lanes is held inside the staged segment
+            // SEG: only sample indices of this segment count; the other
+            // lanes read at sg_rd, whose taps lie inside the staged segment
             [[maybe_unused]] float us = u;
             if constexpr (SEG) {
               static_assert(KIND == 0, "segments take the general loop");
@@ -732,7 +734,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               const bool ins = it >= sg_lo && it < sg_hi;
               ok = ok && ins;
               mk = ins ? mk : 0.f;
-              us = __builtin_amdgcn_fmed3f(u, (float)sg_lo, (float)sg_hi - 0.5f);
+              us = ins ? u : sg_rd;
             }
             const int i0 = (int)us;  // us >= 0: truncation is floor
             const float f = __builtin_amdgcn_fractf(us);
@@ -1005,7 +1007,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
         __syncthreads();
         const int nb = S.nb;
         if (nb > 0) {
-          run_batch(qs, nb, std::false_type{}, 0, 0);
+          run_batch(qs, nb, std::false_type{}, 0, 0, 0.f);
           qs += nb;
           continue;
         }
@@ -1017,13 +1019,21 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
         {
           const int w0 = S.q_w0[qs], len = S.q_len[qs];
           constexpr int seg_cap = SKEW ? ((CAPE - 2) * 32) / 33 : CAPE;
-          // room for a segment beyond the widest taps' reach (antialias pad <= 258)
-          static_assert(seg_cap > 2 * 259 + 1 + 64, "LDS share too small for segments");
+          // with antialias (pad <= 258) every segment holds at least pad
+          // indices beyond the taps' reach on both sides
+          static_assert(!AA || seg_cap >= 4 * 259 + 64, "LDS share too small for antialias segments");
+          static_assert(seg_cap >= 64, "LDS share too small for segments");
           const int pad_lo = AA ? a.aa_pad + 1 : 0, pad_hi = AA ? a.aa_pad + 2 : 1;
-          const int step = seg_cap - pad_lo - pad_hi;
+          // near-equal segments (none short): n_seg of `step` indices
+          const int cap_idx = seg_cap - pad_lo - pad_hi;
+          const int n_seg = (len + cap_idx - 1) / cap_idx;
+          const int step = (len + n_seg - 1) / n_seg;
           for (int lo = w0; lo < w0 + len; lo += step) {
             const int hi = min(lo + step, w0 + len);
             const int s0 = max(lo - pad_lo, w0), s1 = min(hi + pad_hi, w0 + len);
+            // read position of the lanes outside the segment: its taps (the
+            // next sample; with antialias +-(aa_pad + 1)) stay in [s0, s1)
+            const float rd = (float)(AA ? s0 + pad_lo : lo);
             __syncthreads();  // every thread has read the window before it is rewritten
             if (tid == 0) {
               S.q_w0[qs] = s0;
@@ -1032,7 +1042,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               S.b_base[0] = SKEW ? 1 - (s0 + (s0 >> 5)) : -s0;
             }
             __syncthreads();
-            run_batch(qs, 1, std::true_type{}, lo, hi);
+            run_batch(qs, 1, std::true_type{}, lo, hi, rd);
           }
         }
         qs += 1;
