@@ -28,6 +28,22 @@ int launch_adjoint(const pstm_params_t* p, const PlanLayout& L, const TileArgs& 
              : PSTM_ADJ(TX, TY, true, false, GENV, 2, true))          \
       : (vti ? PSTM_ADJ(TX, TY, false, true, GENV, OCCV, FXV)         \
              : PSTM_ADJ(TX, TY, false, false, GENV, OCCV, FXV)))
+  if (long_traces(p, a)) {
+    // windows may exceed every form's LDS share: general forms at the lowest
+    // occupancy (FP32 windows plain, fixed point with antialias), staging
+    // such windows in segments
+#define PSTM_ADJS(TX, TY)                                                                                          \
+  (aa ? (vti ? launch_one<MODE_ADJ, TX, TY, PSTM_F32, true, true, true, 2, true, false, PSTM_WEIGHT_FULL, true>(a, s)    \
+             : launch_one<MODE_ADJ, TX, TY, PSTM_F32, true, false, true, 2, true, false, PSTM_WEIGHT_FULL, true>(a, s))   \
+      : (vti ? launch_one<MODE_ADJ, TX, TY, PSTM_F32, false, true, true, 3, false, false, PSTM_WEIGHT_FULL, true>(a, s)  \
+             : launch_one<MODE_ADJ, TX, TY, PSTM_F32, false, false, true, 3, false, false, PSTM_WEIGHT_FULL, true>(a, s)))
+    if (a.tile_x == 4 && a.tile_y == 2) rc = PSTM_ADJS(4, 2);
+    else if (a.tile_x == 8 && a.tile_y == 1) rc = PSTM_ADJS(8, 1);
+    else if (a.tile_x == 1 && a.tile_y == 8) rc = PSTM_ADJS(1, 8);
+    else rc = set_error(PSTM_ERR_INVALID, "unsupported tile shape %dx%d", a.tile_x, a.tile_y);
+#undef PSTM_ADJS
+    return rc != PSTM_OK ? rc : check_launch("adjoint");
+  }
   // plain fixed-point windows are bank-skewed (skew_len elements)
   const int64_t fxwin = aa ? win : skew_len((int)win);
   const bool fx8 = fxwin <= window_cap_elems<8, false, 3, true>();

@@ -264,7 +264,7 @@ __device__ __forceinline__ float offset_taper(float off, const TileArgs& a) {
 // WM: the weight mode compiled into the fast forms (GEN = false): PSTM_WEIGHT_FULL
 // or PSTM_WEIGHT_SPREADING (the class API's default); the general forms read it.
 template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN, int OCC, bool FIXED, bool V1,
-          int WM = PSTM_WEIGHT_FULL>
+          int WM = PSTM_WEIGHT_FULL, bool CAN_SEG = false>
 __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   constexpr int NC = TX * TY;
   constexpr bool FX = MODE == MODE_ADJ && FIXED;
@@ -288,6 +288,10 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   // a contribution reads one 8-byte element and interpolates with one FMA
   // (the same FP32 difference the single-sample layout forms per read)
   constexpr bool PAIR = MODE == MODE_FWD && !AA && !GEN && !VTI && OCC >= 4;
+  // CAN_SEG: the form launched for traces whose whole window exceeds the
+  // LDS share of the lowest-occupancy form; it stages longer windows in
+  // segments (a separate instantiation, so the other forms carry none of it).
+  static_assert(!CAN_SEG || GEN, "segmenting forms are general forms");
   constexpr int CAPF = sample_cap_floats<NC, AA, OCC>();         // staged-window floats
   constexpr int CAPE = ILV ? CAPF / 4 : PAIR ? CAPF / 2 : window_cap_elems<NC, AA, OCC, FX>();  // staged-window elements
   constexpr int SA = ILV ? 2 : 1;                                // element stride in doubles (Wa)
@@ -541,9 +545,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
       // [sg_lo, sg_hi) are summed (every contribution of the trace belongs to
       // exactly one segment), and every lane's read position is held inside
       // the staged segment.
-      auto run_batch = [&](const int qs, const int nb, auto seg_tag, const int sg_lo, const int sg_hi,
+      auto run_batch = [&](const int qs, const int nb, const bool seg, const int sg_lo, const int sg_hi,
                            const float sg_rd) {
-        constexpr bool SEG = decltype(seg_tag)::value;
         (void)sg_lo;
         (void)sg_hi;
         (void)sg_rd;
@@ -663,7 +666,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
 
         // -- contributions of the traces that reach this wave's samples
         auto trace_body = [&](const int b, const int base, const float2 (&h2v)[NC], const float (&hmv)[NC],
-                              auto full_tag, auto narrow_tag) {
+                              auto full_tag, auto narrow_tag, auto seg_tag) {
+          constexpr bool SEG = decltype(seg_tag)::value;
           // kind 1 (FULL): the trace's times stay inside the record in this
           // wave (no clamp); in the forward it also passes every mask.  Kind 2
           // (forward): it passes every mask but may clamp.  Kind 0: general.
@@ -864,7 +868,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
         // Walk the set bits of a trace mask; the next trace's distances are
         // read from LDS before the current trace is summed, so their latency
         // overlaps the arithmetic.
-        auto walk = [&](uint64_t m, auto full_tag, auto narrow_tag) {
+        auto walk = [&](uint64_t m, auto full_tag, auto narrow_tag, auto seg_tag) {
           if constexpr (!PREFETCH) {
             for (; m != 0; m &= m - 1) {
               const int b = uniform_i(__builtin_ctzll(m));
@@ -875,7 +879,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
                 h2c[c] = P.h2[b][c];
                 hmc[c] = P.hm[b][c];
               }
-              trace_body(b, uniform_i(S.b_base[b]), h2c, hmc, full_tag, narrow_tag);
+              trace_body(b, uniform_i(S.b_base[b]), h2c, hmc, full_tag, narrow_tag, seg_tag);
             }
             return;
           }
@@ -901,7 +905,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               hmn[c] = P.hm[bn][c];
             }
             const int basen = uniform_i(S.b_base[bn]);
-            trace_body(b, basec, h2c, hmc, full_tag, narrow_tag);
+            trace_body(b, basec, h2c, hmc, full_tag, narrow_tag, seg_tag);
             if (!more) break;
             b = bn;
             basec = basen;
@@ -914,30 +918,32 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
         };
         const uint64_t todo = __ballot(lane < nb && ((P.act[lane] >> wave) & 1));
         uint64_t tfull = 0;
-        if constexpr (SEG) {
-          if (!SKEW || narrow) walk(todo, std::integral_constant<int, 0>{}, std::true_type{});
-          else walk(todo, std::integral_constant<int, 0>{}, std::false_type{});
+        if (CAN_SEG && seg) {
+          if constexpr (CAN_SEG) {
+            if (!SKEW || narrow) walk(todo, std::integral_constant<int, 0>{}, std::true_type{}, std::true_type{});
+            else walk(todo, std::integral_constant<int, 0>{}, std::false_type{}, std::true_type{});
+          }
         } else if constexpr (FULLPATH) {
           tfull = __ballot(lane < nb && ((P.act[lane] >> (4 + wave)) & 1));
           const uint64_t tunm = __ballot(lane < nb && ((P.act[lane] >> (8 + wave)) & 1));
           nfull += (int)__popcll(tfull) + (int)__popcll(tunm);
-          walk(tfull, std::integral_constant<int, 1>{}, std::true_type{});
-          walk(tunm, std::integral_constant<int, 2>{}, std::true_type{});
+          walk(tfull, std::integral_constant<int, 1>{}, std::true_type{}, std::false_type{});
+          walk(tunm, std::integral_constant<int, 2>{}, std::true_type{}, std::false_type{});
           tfull |= tunm;
         }
-        if constexpr (SEG) {
+        if (CAN_SEG && seg) {
         } else if constexpr (MODE == MODE_ADJ && !AA) {
           // traces inside the record in this wave, then those that clamp
           tfull = __ballot(lane < nb && ((P.act[lane] >> (4 + wave)) & 1));
           if (!SKEW || narrow) {
-            walk(tfull, std::integral_constant<int, 1>{}, std::true_type{});
-            walk(todo & ~tfull, std::integral_constant<int, 0>{}, std::true_type{});
+            walk(tfull, std::integral_constant<int, 1>{}, std::true_type{}, std::false_type{});
+            walk(todo & ~tfull, std::integral_constant<int, 0>{}, std::true_type{}, std::false_type{});
           } else {
-            walk(tfull, std::integral_constant<int, 1>{}, std::false_type{});
-            walk(todo & ~tfull, std::integral_constant<int, 0>{}, std::false_type{});
+            walk(tfull, std::integral_constant<int, 1>{}, std::false_type{}, std::false_type{});
+            walk(todo & ~tfull, std::integral_constant<int, 0>{}, std::false_type{}, std::false_type{});
           }
         } else {
-          walk(todo & ~tfull, std::integral_constant<int, 0>{}, std::true_type{});
+          walk(todo & ~tfull, std::integral_constant<int, 0>{}, std::true_type{}, std::false_type{});
         }
         if constexpr (MODE != MODE_ADJ) {
           nsum += __popcll(todo);
@@ -985,67 +991,89 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           __syncthreads();
         }
       };
+      // A window longer than the LDS share (only possible in the forms a
+      // dispatch gives traces longer than every other form's share: CAN_SEG)
+      // is processed in segments of at most seg_cap staged elements, each
+      // assigned the sample indices [lo, hi) and staged with the taps' reach
+      // on either side (the next sample; with antialias the triangle's
+      // half-width and its neighbour), clipped to the window.  One call site
+      // of run_batch serves both kinds of sub-batch.
+      constexpr int seg_cap = SKEW ? ((CAPE - 2) * 32) / 33 : CAPE;
+      // with antialias (pad <= 258) every segment holds at least pad indices
+      // beyond the taps' reach on both sides
+      static_assert(!CAN_SEG || !AA || seg_cap >= 4 * 259 + 64, "LDS share too small for antialias segments");
+      static_assert(!CAN_SEG || seg_cap >= 64, "LDS share too small for segments");
+      bool segmenting = false;
+      int seg_w0 = 0, seg_lo = 0, seg_step = 0, seg_end = 0;
+      const int pad_lo = AA ? a.aa_pad + 1 : 0, pad_hi = AA ? a.aa_pad + 2 : 1;
       while (true) {
-        const int rem = qn - qs;
-        if (rem <= 0 || (!final_flush && rem < MB)) break;
-        if (wave == 0) {
-          const int i = qs + lane;
-          const bool in = lane < MB && i < qn;
-          const int cost = in ? (SKEW ? skew_len(S.q_len[i]) : S.q_len[i]) : 0;  // window elements
-          const int incl = wave_scan(cost, lane);
-          const bool fit = in && incl <= CAPE;
-          const uint64_t m = __ballot(fit);
-          if (fit) {
-            const int off = incl - cost;
-            const int w0 = S.q_w0[i];
-            S.b_off[lane] = off;
-            // SKEW: sample i at base + i + (i >> 5), one spare slot before the window (w0 >= 0)
-            S.b_base[lane] = SKEW ? off + 1 - (w0 + (w0 >> 5)) : off - w0;
-          }
-          if (lane == 0) S.nb = __popcll(m);
-        }
-        __syncthreads();
-        const int nb = S.nb;
-        if (nb > 0) {
-          run_batch(qs, nb, std::false_type{}, 0, 0, 0.f);
-          qs += nb;
-          continue;
-        }
-        // The first window alone exceeds the LDS share: it is processed in
-        // segments of at most seg_cap staged elements, each assigned the
-        // sample indices [lo, hi) and staged with the taps' reach on either
-        // side (the next sample; with antialias the triangle's half-width
-        // and its neighbour), clipped to the window.
-        {
-          const int w0 = S.q_w0[qs], len = S.q_len[qs];
-          constexpr int seg_cap = SKEW ? ((CAPE - 2) * 32) / 33 : CAPE;
-          // with antialias (pad <= 258) every segment holds at least pad
-          // indices beyond the taps' reach on both sides
-          static_assert(!AA || seg_cap >= 4 * 259 + 64, "LDS share too small for antialias segments");
-          static_assert(seg_cap >= 64, "LDS share too small for segments");
-          const int pad_lo = AA ? a.aa_pad + 1 : 0, pad_hi = AA ? a.aa_pad + 2 : 1;
-          // near-equal segments (none short): n_seg of `step` indices
-          const int cap_idx = seg_cap - pad_lo - pad_hi;
-          const int n_seg = (len + cap_idx - 1) / cap_idx;
-          const int step = (len + n_seg - 1) / n_seg;
-          for (int lo = w0; lo < w0 + len; lo += step) {
-            const int hi = min(lo + step, w0 + len);
-            const int s0 = max(lo - pad_lo, w0), s1 = min(hi + pad_hi, w0 + len);
-            // read position of the lanes outside the segment: its taps (the
-            // next sample; with antialias +-(aa_pad + 1)) stay in [s0, s1)
-            const float rd = (float)(AA ? s0 + pad_lo : lo);
-            __syncthreads();  // every thread has read the window before it is rewritten
-            if (tid == 0) {
-              S.q_w0[qs] = s0;
-              S.q_len[qs] = s1 - s0;
-              S.b_off[0] = 0;
-              S.b_base[0] = SKEW ? 1 - (s0 + (s0 >> 5)) : -s0;
+        int nb = 0;
+        if (!segmenting) {
+          const int rem = qn - qs;
+          if (rem <= 0 || (!final_flush && rem < MB)) break;
+          if (wave == 0) {
+            const int i = qs + lane;
+            const bool in = lane < MB && i < qn;
+            const int cost = in ? (SKEW ? skew_len(S.q_len[i]) : S.q_len[i]) : 0;  // window elements
+            const int incl = wave_scan(cost, lane);
+            const bool fit = in && incl <= CAPE;
+            const uint64_t m = __ballot(fit);
+            if (fit) {
+              const int off = incl - cost;
+              const int w0 = S.q_w0[i];
+              S.b_off[lane] = off;
+              // SKEW: sample i at base + i + (i >> 5), one spare slot before the window (w0 >= 0)
+              S.b_base[lane] = SKEW ? off + 1 - (w0 + (w0 >> 5)) : off - w0;
             }
-            __syncthreads();
-            run_batch(qs, 1, std::true_type{}, lo, hi, rd);
+            if (lane == 0) S.nb = __popcll(m);
+          }
+          __syncthreads();
+          nb = S.nb;
+          if (nb == 0) {
+            if constexpr (!CAN_SEG) {
+              qs += 1;  // not reached: the dispatch gives these forms windows that fit
+              continue;
+            }
+            // near-equal segments (none short) of `seg_step` indices
+            const int w0 = S.q_w0[qs], len = S.q_len[qs];
+            const int cap_idx = seg_cap - pad_lo - pad_hi;
+            const int n_seg = (len + cap_idx - 1) / cap_idx;
+            seg_step = (len + n_seg - 1) / n_seg;
+            seg_w0 = w0;
+            seg_lo = w0;
+            seg_end = w0 + len;
+            segmenting = true;
           }
         }
-        qs += 1;
+        int lo = 0, hi = 0;
+        float rd = 0.f;
+        if (CAN_SEG && segmenting) {
+          lo = seg_lo;
+          hi = min(lo + seg_step, seg_end);
+          const int s0 = max(lo - pad_lo, seg_w0), s1 = min(hi + pad_hi, seg_end);
+          // read position of the lanes outside the segment: its taps (the
+          // next sample; with antialias +-(aa_pad + 1)) stay in [s0, s1)
+          rd = (float)(AA ? s0 + pad_lo : lo);
+          __syncthreads();  // every thread has read the entry before it is rewritten
+          if (tid == 0) {
+            S.q_w0[qs] = s0;
+            S.q_len[qs] = s1 - s0;
+            S.b_off[0] = 0;
+            S.b_base[0] = SKEW ? 1 - (s0 + (s0 >> 5)) : -s0;
+          }
+          __syncthreads();
+          nb = 1;
+        }
+        run_batch(qs, nb, CAN_SEG && segmenting, lo, hi, rd);
+        if (CAN_SEG && segmenting) {
+          seg_lo = hi;
+          if (seg_lo >= seg_end) {
+            segmenting = false;
+            qs += 1;
+          }
+        } else {
+          qs += nb;
+        }
       }
       // move the unprocessed tail (< MB entries) to the front
       const int rem = qn - qs;
@@ -1183,7 +1211,7 @@ inline int64_t tile_blocks(const TileArgs& a) {
 }
 
 template <int MODE, int TX, int TY, int DT, bool AA, bool VTI, bool GEN, int OCC = default_occ(AA),
-          bool FIXED = false, bool V1 = false, int WM = PSTM_WEIGHT_FULL>
+          bool FIXED = false, bool V1 = false, int WM = PSTM_WEIGHT_FULL, bool SEG = false>
 int launch_one(const TileArgs& a, hipStream_t s) {
   static_assert(!GEN || WM == PSTM_WEIGHT_FULL, "the general forms read the weight mode");
   const int64_t blocks = tile_blocks(a);
@@ -1194,8 +1222,8 @@ int launch_one(const TileArgs& a, hipStream_t s) {
   for (int band = a.band0; band < a.band1; ++band) {
     b.band0 = band;
     b.band1 = band + 1;
-    hipLaunchKernelGGL((k_tile<MODE, TX, TY, DT, AA, VTI, GEN, OCC, FIXED, V1, WM>), dim3((unsigned)blocks), dim3(kThreads),
-                       0, s, b);
+    hipLaunchKernelGGL((k_tile<MODE, TX, TY, DT, AA, VTI, GEN, OCC, FIXED, V1, WM, SEG>), dim3((unsigned)blocks),
+                       dim3(kThreads), 0, s, b);
   }
   return PSTM_OK;
 }
@@ -1207,6 +1235,15 @@ inline bool fast_path(const pstm_params_t* p) {
 // The plain forward also has fast forms for the SPREADING weight.
 inline bool fast_path_spreading(const pstm_params_t* p) {
   return p->weight_mode == PSTM_WEIGHT_SPREADING && !(p->flags & (PSTM_FLAG_TAPER | PSTM_FLAG_ZERO_OUTSIDE));
+}
+
+// Whether a call's whole trace window (every sample plus the antialias
+// padding) may exceed the LDS share of the lowest-occupancy forms (3
+// workgroups per CU plain, 2 with antialias; 8-column tiles): such calls run
+// the segmenting general forms.
+inline bool long_traces(const pstm_params_t* p, const TileArgs& a) {
+  const bool aa = (p->flags & PSTM_FLAG_AA) != 0;
+  return whole_window(p, a.aa_pad) > min_cap_elems(aa);
 }
 
 // Per-dtype forward dispatch (defined in pstm_fwd_<dtype>.hip).
@@ -1223,6 +1260,20 @@ template <int DT>
 int launch_forward_impl(const pstm_params_t* p, const PlanLayout& L, const TileArgs& a, hipStream_t s) {
   const bool aa = (p->flags & PSTM_FLAG_AA) != 0, vti = (p->flags & PSTM_FLAG_VTI) != 0;
   const bool gen = !fast_path(p);
+  if (long_traces(p, a)) {
+    // windows may exceed every form's LDS share: the general form at the
+    // lowest occupancy, staging such windows in segments
+#define PSTM_FWDS(TX, TY)                                                                                     \
+  (aa ? (vti ? launch_one<MODE_FWD, TX, TY, DT, true, true, true, 2, false, false, PSTM_WEIGHT_FULL, true>(a, s)    \
+             : launch_one<MODE_FWD, TX, TY, DT, true, false, true, 2, false, false, PSTM_WEIGHT_FULL, true>(a, s))   \
+      : (vti ? launch_one<MODE_FWD, TX, TY, DT, false, true, true, 3, false, false, PSTM_WEIGHT_FULL, true>(a, s)   \
+             : launch_one<MODE_FWD, TX, TY, DT, false, false, true, 3, false, false, PSTM_WEIGHT_FULL, true>(a, s)))
+    if (a.tile_x == 4 && a.tile_y == 2) return PSTM_FWDS(4, 2);
+    if (a.tile_x == 8 && a.tile_y == 1) return PSTM_FWDS(8, 1);
+    if (a.tile_x == 1 && a.tile_y == 8) return PSTM_FWDS(1, 8);
+#undef PSTM_FWDS
+    return set_error(PSTM_ERR_INVALID, "unsupported tile shape %dx%d", a.tile_x, a.tile_y);
+  }
 #define PSTM_FWD(TX, TY, AAV, VTIV, GENV) launch_one<MODE_FWD, TX, TY, DT, AAV, VTIV, GENV>(a, s)
 #define PSTM_FWD4(TX, TY, GENV)                         \
   (aa ? (vti ? PSTM_FWD(TX, TY, true, true, GENV)       \
