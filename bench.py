@@ -200,7 +200,7 @@ def _workload(args, torch):
                "image": [s.params.n_il, s.params.n_xl, s.params.n_tau], "bin_m": 25.0,
                "aperture_m": 3000.0, "antialias": True,
                "velocity": "v_rms = (1500 + 600 t)(1 +- 5% lateral)"}
-        forms = ["fp32", "bf16"]
+        forms = ["fp32", "fp16", "bf16"]
     else:
         # --classes n: n classes over the same 0-4000 m offset range (a launch must end within minutes)
         s = S.vti_largest_3d(device=dev, n_classes=args.classes or 60)
