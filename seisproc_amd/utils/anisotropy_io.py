@@ -157,3 +157,22 @@ def write_anisotropy_auto(model: AnisotropyModel, filepath: Union[str, Path], **
         write_anisotropy_npy(model, path)
     else:
         write_anisotropy_json(model, path, **kwargs)
+
+
+def create_anisotropy_from_picks(picks, dz: float = 0.004, z_max: Optional[float] = None) -> AnisotropyModel:
+    """(z, epsilon, delta) picks -> epsilon(z), delta(z) on 0, dz, ..., z_max
+    by linear interpolation, constant beyond the end picks (reference
+    utils/anisotropy_io.py:395-437); eta follows from epsilon and delta."""
+    if not picks:
+        raise ValueError("No anisotropy picks provided")
+    picks = sorted(picks, key=lambda x: x[0])
+    zp = np.array([p[0] for p in picks], dtype=np.float64)
+    ep = np.array([p[1] for p in picks], dtype=np.float64)
+    dp = np.array([p[2] for p in picks], dtype=np.float64)
+    z_max = float(zp[-1]) if z_max is None else float(z_max)
+    n = int(np.floor(z_max / dz + 1e-9)) + 1
+    z = dz * np.arange(n)
+    m = AnisotropyModel(epsilon=np.interp(z, zp, ep).astype(np.float32), delta=np.interp(z, zp, dp).astype(np.float32),
+                        z_axis=z)
+    m.metadata.update(source="picks", n_picks=len(picks))
+    return m

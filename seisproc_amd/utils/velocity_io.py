@@ -19,16 +19,22 @@ is a plain string array, so nothing needs unpickling); the lateral axes of
 CDP and inline/xline tables are kept in those numbers and the model records
 it (``metadata['lateral_axes']``) so that ``VelocityModel.vrms_on_grid``
 samples them at the image columns' CDP or inline/xline numbers.  SEG-Y
-velocity cubes are out of scope.
+velocity files are read by the hand-written reader of ``utils/segy.py`` (the
+reference uses segyio, :1096-1174): one trace is a v(t); several traces are
+v(t, CDP) as in the reference, or, when their inline/crossline headers
+cover a full lattice, a v(t, inline, xline) cube.  ``preview_velocity_file``
+/ ``velocity_info_to_model`` (:166-512, :1175-1290) and
+``create_velocity_from_picks`` (:953-1006) follow the reference.
 """
 from __future__ import annotations
 
 import json
 import logging
 import re
+from dataclasses import dataclass, field
 from enum import Enum
 from pathlib import Path
-from typing import Optional, Union
+from typing import Any, Dict, List, Optional, Tuple, Union
 
 import numpy as np
 
@@ -290,8 +296,7 @@ def read_velocity_auto(filepath: Union[str, Path], **kwargs) -> VelocityModel:
     if suf in (".npy", ".npz"):
         return read_velocity_npy(path)
     if suf in (".sgy", ".segy"):
-        raise NotImplementedError("SEG-Y velocity cubes are not read here; load the cube with a SEG-Y "
-                                  "reader and pass it as a 3D VelocityModel or a device v_rms array")
+        return read_velocity_segy(path, velocity_type=kwargs.get("velocity_type", "rms"))
     fmt = detect_velocity_format(path)
     units = {k: kwargs[k] for k in ("time_unit", "velocity_unit", "velocity_type") if k in kwargs}
     if fmt is VelocityFileFormat.ASCII_CDPTV:
@@ -314,3 +319,213 @@ def write_velocity_auto(model: VelocityModel, filepath: Union[str, Path], **kwar
         if suf != ".json":
             logger.warning("Unknown extension %s, using JSON format", suf)
         write_velocity_json(model, path, **kwargs)
+
+
+# ---------------------------------------------------------------------------
+# SEG-Y, preview and picks
+
+def read_velocity_segy(filepath: Union[str, Path], velocity_type: str = "rms", inline_byte: Optional[int] = None,
+                       xline_byte: Optional[int] = None) -> VelocityModel:
+    """SEG-Y velocity file (reference utils/velocity_io.py:1096-1174): each
+    trace is a velocity function at regular times (the sample interval and
+    the first trace's delay time).  One trace -> v(t); several traces whose
+    inline/crossline headers (bytes 189/193, or inline_byte/xline_byte) form a
+    full lattice of more than one inline and crossline -> v(t, inline,
+    xline) with ``lateral_axes='inline_xline'``; otherwise -> v(t, CDP) with
+    the CDP numbers (trace numbers when every CDP header is 0) as lateral
+    axis, as the reference builds it."""
+    from .segy import read_segy
+    path = Path(filepath)
+    if not path.exists():
+        raise FileNotFoundError(f"Velocity file not found: {path}")
+    vol = read_segy(path, inline_byte, xline_byte)
+    if vol.n_traces == 0:
+        raise ValueError(f"{path}: no traces")
+    times = vol.times_s.astype(np.float64)
+    md = {"source_file": str(path), "velocity_type": velocity_type, "format": "segy",
+          "sample_format": vol.format_code}
+    if vol.n_traces == 1:
+        return VelocityModel(data=vol.samples[0].astype(np.float32), z_axis=times, is_time=True, metadata=md)
+    il, xl = vol.headers["inline"], vol.headers["crossline"]
+    uil, uxl = np.unique(il), np.unique(xl)
+    if (np.all(il != 0) and np.all(xl != 0) and uil.size > 1 and uxl.size > 1
+            and uil.size * uxl.size == vol.n_traces
+            and np.unique(il * (int(xl.max()) + 1) + xl).size == vol.n_traces):
+        cube = np.empty((times.size, uil.size, uxl.size), dtype=np.float32)
+        cube[:, np.searchsorted(uil, il), np.searchsorted(uxl, xl)] = vol.samples.T
+        md.update(lateral_axes="inline_xline", inline_range=(int(uil.min()), int(uil.max())),
+                  xline_range=(int(uxl.min()), int(uxl.max())))
+        return VelocityModel(data=cube, z_axis=times, x_axis=uil.astype(np.float64), y_axis=uxl.astype(np.float64),
+                             is_time=True, metadata=md)
+    cdp = vol.headers["cdp"]
+    x = cdp.astype(np.float64) if np.any(cdp != 0) else np.arange(vol.n_traces, dtype=np.float64)
+    order = np.argsort(x, kind="stable")
+    if np.unique(x).size != x.size:
+        raise ValueError(f"{path}: repeated CDP numbers and no inline/crossline lattice")
+    md.update(lateral_axes="cdp", cdp_range=(int(cdp.min()), int(cdp.max())) if np.any(cdp != 0) else None)
+    return VelocityModel(data=vol.samples[order].T.astype(np.float32), z_axis=times, x_axis=x[order], is_time=True,
+                         metadata=md)
+
+
+@dataclass
+class VelocityFileInfo:
+    """What a velocity file holds, for preview (reference
+    utils/velocity_io.py:59-91): sizes, ranges and the parsed arrays
+    (``raw_data``) that velocity_info_to_model turns into a model."""
+
+    path: Path
+    format: VelocityFileFormat = VelocityFileFormat.UNKNOWN
+    n_locations: int = 0
+    n_time_samples: int = 0
+    time_range: Tuple[float, float] = (0.0, 0.0)
+    velocity_range: Tuple[float, float] = (0.0, 0.0)
+    cdp_range: Optional[Tuple[int, int]] = None
+    inline_range: Optional[Tuple[int, int]] = None
+    xline_range: Optional[Tuple[int, int]] = None
+    sample_interval: Optional[float] = None
+    is_valid: bool = False
+    error_message: Optional[str] = None
+    raw_data: Dict[str, Any] = field(default_factory=dict)
+
+
+def _regular_interval(t: np.ndarray) -> Optional[float]:
+    d = np.diff(np.unique(t))
+    return float(d[0]) if d.size and np.allclose(d, d[0], rtol=1e-6) else None
+
+
+def preview_velocity_file(filepath: Union[str, Path], inline_byte: Optional[int] = None,
+                          xline_byte: Optional[int] = None) -> VelocityFileInfo:
+    """Detect the format and parse the file without unit conversion (times
+    and velocities as written; SEG-Y times in s).  Errors are reported in
+    ``error_message`` with ``is_valid`` False, never raised."""
+    path = Path(filepath)
+    info = VelocityFileInfo(path=path)
+    if not path.exists():
+        info.error_message = f"File not found: {path}"
+        return info
+    info.format = detect_velocity_format(path)
+    try:
+        rows = None
+        if info.format in (VelocityFileFormat.ASCII_TV, VelocityFileFormat.ASCII_CDPTV,
+                           VelocityFileFormat.ASCII_ILXLTV):
+            need = {VelocityFileFormat.ASCII_TV: 2, VelocityFileFormat.ASCII_CDPTV: 3}.get(info.format, 4)
+            rows = np.asarray([r[:need] for _, s in _data_lines(path) if len(r := _numbers(s)) >= need])
+            if rows.size == 0:
+                info.error_message = "No valid data found"
+                return info
+        if info.format is VelocityFileFormat.ASCII_TV:
+            t, v = rows[:, 0], rows[:, 1]
+            info.n_locations, info.n_time_samples = 1, t.size
+            info.raw_data = {"times": t, "velocities": v, "format": "tv"}
+        elif info.format is VelocityFileFormat.ASCII_CDPTV:
+            cdp, t, v = rows[:, 0], rows[:, 1], rows[:, 2]
+            u = np.unique(cdp)
+            info.n_locations, info.n_time_samples = u.size, int(np.sum(cdp == u[0]))
+            info.cdp_range = (int(u.min()), int(u.max()))
+            info.raw_data = {"cdps": cdp, "times": t, "velocities": v, "unique_cdps": u, "format": "cdp_tv"}
+        elif info.format is VelocityFileFormat.ASCII_ILXLTV:
+            il, xl, t, v = rows[:, 0], rows[:, 1], rows[:, 2], rows[:, 3]
+            uil, uxl = np.unique(il), np.unique(xl)
+            info.n_locations = int(np.unique(np.stack([il, xl], 1), axis=0).shape[0])
+            info.n_time_samples = int(np.sum((il == il[0]) & (xl == xl[0])))
+            info.inline_range, info.xline_range = (int(uil.min()), int(uil.max())), (int(uxl.min()), int(uxl.max()))
+            info.raw_data = {"inlines": il, "xlines": xl, "times": t, "velocities": v, "unique_inlines": uil,
+                             "unique_xlines": uxl, "format": "ilxl_tv"}
+        elif info.format is VelocityFileFormat.SEGY:
+            from .segy import read_segy
+            vol = read_segy(path, inline_byte, xline_byte)
+            h = vol.headers
+            t, v = vol.times_s, vol.samples
+            info.n_locations, info.n_time_samples = vol.n_traces, t.size
+            info.sample_interval = vol.interval_s
+            for key, name in (("cdp", "cdp_range"), ("inline", "inline_range"), ("crossline", "xline_range")):
+                if np.any(h[key] != 0):
+                    setattr(info, name, (int(h[key].min()), int(h[key].max())))
+            info.raw_data = {"format": "segy", "filepath": str(path), "inline_byte": inline_byte,
+                             "xline_byte": xline_byte}
+        elif info.format is VelocityFileFormat.JSON:
+            m = read_velocity_json(path)
+            t, v = np.atleast_1d(m.z_axis if m.z_axis is not None else [0.0]), np.atleast_1d(m.data)
+            info.n_locations = int(np.prod(np.shape(m.data)[1:])) if np.ndim(m.data) > 1 else 1
+            info.n_time_samples = int(np.shape(m.data)[0]) if np.ndim(m.data) else 1
+            info.raw_data = {"format": "json", "data": m.to_dict()}
+        elif info.format is VelocityFileFormat.NPZ:
+            m = read_velocity_npy(path)
+            t, v = np.atleast_1d(m.z_axis if m.z_axis is not None else [0.0]), np.atleast_1d(m.data)
+            info.n_locations = int(np.prod(np.shape(m.data)[1:])) if np.ndim(m.data) > 1 else 1
+            info.n_time_samples = int(np.shape(m.data)[0]) if np.ndim(m.data) else 1
+            info.raw_data = {"format": "npz", "filepath": str(path)}
+        else:
+            info.error_message = "Unknown file format"
+            return info
+        info.time_range = (float(np.min(t)), float(np.max(t)))
+        info.velocity_range = (float(np.min(v)), float(np.max(v)))
+        if info.sample_interval is None:
+            info.sample_interval = _regular_interval(np.asarray(t, dtype=np.float64))
+        info.is_valid = True
+    except Exception as e:  # the preview reports, the caller decides
+        info.error_message = f"Preview error: {e}"
+    return info
+
+
+def velocity_info_to_model(info: VelocityFileInfo, velocity_type: str = "rms", time_unit: str = "ms",
+                           velocity_unit: str = "m/s") -> VelocityModel:
+    """Turn a preview into a model (reference :1175-1290): text formats with
+    the given units (time-velocity -> v(t); CDP tables -> v(t, CDP);
+    inline-xline tables -> v(t, inline) on the first inline's time axis, as
+    the reference builds it); SEG-Y, JSON and .npz through their readers."""
+    if not info.is_valid:
+        raise ValueError(f"Invalid velocity file: {info.error_message}")
+    raw = info.raw_data
+    fmt = raw.get("format", "")
+    tdiv = _scale(time_unit, _TIME_DIV, "time")
+    vmul = _scale(velocity_unit, _VEL_SCALE, "velocity")
+    if fmt == "tv":
+        t = np.asarray(raw["times"], dtype=np.float64) / tdiv
+        v = np.asarray(raw["velocities"], dtype=np.float64) * vmul
+        o = np.argsort(t, kind="stable")
+        return VelocityModel(data=v[o].astype(np.float32), z_axis=t[o], is_time=True,
+                             metadata={"source_file": str(info.path), "velocity_type": velocity_type})
+    if fmt == "cdp_tv":
+        return read_velocity_cdptv(info.path, time_unit=time_unit, velocity_unit=velocity_unit,
+                                   velocity_type=velocity_type)
+    if fmt == "ilxl_tv":
+        il = np.asarray(raw["inlines"])
+        t = np.asarray(raw["times"], dtype=np.float64) / tdiv
+        v = np.asarray(raw["velocities"], dtype=np.float64) * vmul
+        uil = np.asarray(raw["unique_inlines"])
+        t_axis = np.sort(t[il == uil[0]])
+        vel = np.empty((t_axis.size, uil.size), dtype=np.float32)
+        for i, u in enumerate(uil):
+            m = il == u
+            o = np.argsort(t[m])
+            vel[:, i] = np.interp(t_axis, t[m][o], v[m][o])
+        return VelocityModel(data=vel, z_axis=t_axis, x_axis=uil.astype(np.float64), is_time=True,
+                             metadata={"source_file": str(info.path), "velocity_type": velocity_type,
+                                       "lateral_axes": "cdp", "inline_range": info.inline_range,
+                                       "xline_range": info.xline_range})
+    if fmt == "segy":
+        return read_velocity_segy(info.path, velocity_type=velocity_type, inline_byte=raw.get("inline_byte"),
+                                  xline_byte=raw.get("xline_byte"))
+    if fmt == "json":
+        return read_velocity_json(info.path)
+    if fmt == "npz":
+        return read_velocity_npy(info.path)
+    raise ValueError(f"Unknown velocity format: {fmt}")
+
+
+def create_velocity_from_picks(picks: List[Tuple[float, float]], dt: float = 0.004, t_max: Optional[float] = None,
+                               velocity_type: str = "rms") -> VelocityModel:
+    """(time s, velocity m/s) picks -> v(t) on 0, dt, ..., t_max by linear
+    interpolation, constant beyond the first and last pick (reference
+    :953-1006)."""
+    if not picks:
+        raise ValueError("No velocity picks provided")
+    picks = sorted(picks, key=lambda x: x[0])
+    tp = np.array([p[0] for p in picks], dtype=np.float64)
+    vp = np.array([p[1] for p in picks], dtype=np.float64)
+    t_max = float(tp[-1]) if t_max is None else float(t_max)
+    n = int(np.floor(t_max / dt + 1e-9)) + 1
+    t_axis = dt * np.arange(n)
+    return VelocityModel(data=np.interp(t_axis, tp, vp).astype(np.float32), z_axis=t_axis, is_time=True,
+                         metadata={"velocity_type": velocity_type, "source": "picks", "n_picks": len(picks)})

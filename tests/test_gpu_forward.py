@@ -240,3 +240,31 @@ def test_input_validation_errors():
         plan.forward(s.traces.to(DEV)[:, :50], s.vel.to(DEV))  # wrong shape
     with pytest.raises(RuntimeError, match="n_samples"):
         PstmPlan(s.params.replace(n_samples=1), s.src_xy, s.rcv_xy)
+
+
+def test_forward_with_segy_velocity_cube(tmp_path):
+    # a v_rms(t, inline, xline) cube written as SEG-Y (IBM floats), read by the
+    # velocity reader and sampled at the image columns (inline/xline numbers
+    # 1..n), migrates like the oracle with the same table
+    from seisproc_amd.processors.migration.hip_ops import PstmPlan
+    from seisproc_amd.utils.segy import write_segy
+    from seisproc_amd.utils.velocity_io import read_velocity_auto
+    s = S.tiny_survey(seed=21, n_bins=12, n_classes=2, n_samples=251, n_img=8, aperture=350.0)
+    p = s.params
+    t = 0.008 * np.arange(160)  # coarser than the image times: the reader's table is interpolated
+    il, xl = np.meshgrid(np.arange(1, 9), np.arange(1, 9), indexing="ij")
+    v = 1700.0 + 450.0 * t[None, :] + 15.0 * il.ravel()[:, None] - 10.0 * xl.ravel()[:, None]
+    f = tmp_path / "vrms.sgy"
+    write_segy(f, v, 0.008, format_code=1, inline=il.ravel(), crossline=xl.ravel())
+    model = read_velocity_auto(f)
+    assert model.metadata["lateral_axes"] == "inline_xline"
+    vel = model.vrms_on_grid(p, device=DEV)
+    assert tuple(vel.shape) == (8, 8, p.n_tau)
+    tau = p.tau0 + p.dtau * np.arange(p.n_tau)
+    want = 1700.0 + 450.0 * np.minimum(tau, t[-1])[None, None, :] + 15.0 * il[..., None] - 10.0 * xl[..., None]
+    np.testing.assert_allclose(vel.cpu().numpy(), want, rtol=1e-5)
+    pp = p.replace(vel_mode=1)
+    img, fold = PstmPlan(pp, s.src_xy, s.rcv_xy).forward(s.traces.to(DEV), vel)
+    sx, sy, rx, ry = s.numpy_geometry()
+    ref, rfold = oracle.forward(pp, s.traces.double().numpy(), sx, sy, rx, ry, vel.double().cpu().numpy())
+    _compare("segy-velocity-cube", img, fold, ref, rfold)

@@ -93,3 +93,102 @@ def test_anisotropy_readers(tmp_path):
     assert aio.read_anisotropy_npy(tmp_path / "c.npz").eta == pytest.approx(0.1 / 1.04)
     e = m.eta_on_grid(_grid(n_tau=6, dtau=0.2), device="cpu").numpy()
     np.testing.assert_allclose(e, np.interp(0.2 * np.arange(6), [0.0, 1.0], m.eta), rtol=1e-6)
+
+
+# ---------------------------------------------------------------- SEG-Y
+def test_ibm_float_known_answers():
+    # IBM System/360 hexadecimal floats: 0x42640000 = 100.0, 0xC276A000 =
+    # -118.625, 0x41100000 = 1.0, 0x3F800000 = 0.03125 (= 0.5 x 16^-1)
+    from seisproc_amd.utils.segy import float_to_ibm, ibm_to_float
+    words = np.array([0x42640000, 0xC276A000, 0x41100000, 0x3F800000, 0], dtype=np.uint32)
+    vals = np.array([100.0, -118.625, 1.0, 0.03125, 0.0])
+    np.testing.assert_array_equal(ibm_to_float(words), vals)
+    np.testing.assert_array_equal(float_to_ibm(vals), words)
+    # round trip of velocities and of arbitrary floats to the 24-bit fraction
+    v = np.arange(1400.0, 6000.0, 0.5)
+    np.testing.assert_array_equal(ibm_to_float(float_to_ibm(v)), v)
+    x = np.random.default_rng(0).normal(size=1000) * 1e3
+    assert np.max(np.abs(ibm_to_float(float_to_ibm(x)) - x) / np.abs(x)) < 2 ** -20
+
+
+def _cube(n_il=3, n_xl=4, nt=50):
+    t = 0.004 * np.arange(nt)
+    il, xl = np.meshgrid(np.arange(101, 101 + n_il), np.arange(7, 7 + n_xl), indexing="ij")
+    v = 1500.0 + 800.0 * t[None, None, :] + 10.0 * il[..., None] + xl[..., None]
+    return t, il.ravel(), xl.ravel(), v.reshape(n_il * n_xl, nt)
+
+
+@pytest.mark.parametrize("fmt,little,n_ext", [(1, False, 0), (5, False, 1), (5, True, 0), (2, False, 0)])
+def test_read_velocity_segy_cube(tmp_path, fmt, little, n_ext):
+    from seisproc_amd.utils.segy import write_segy
+    from seisproc_amd.utils.velocity_io import read_velocity_auto, read_velocity_segy
+    t, il, xl, v = _cube()
+    f = tmp_path / "vel.sgy"
+    write_segy(f, v, 0.004, format_code=fmt, inline=il, crossline=xl, cdp=np.arange(1, il.size + 1),
+               little_endian=little, n_extended_text=n_ext)
+    m = read_velocity_segy(f)
+    assert m.metadata["lateral_axes"] == "inline_xline" and m.data.shape == (50, 3, 4)
+    np.testing.assert_allclose(m.z_axis, t, rtol=0, atol=1e-12)
+    np.testing.assert_array_equal(m.x_axis, [101, 102, 103])
+    np.testing.assert_array_equal(m.y_axis, [7, 8, 9, 10])
+    want = v.reshape(3, 4, 50).transpose(2, 0, 1)
+    np.testing.assert_allclose(m.data, np.round(want) if fmt == 2 else want, rtol=0, atol=1e-3)
+    assert read_velocity_auto(f).data.shape == (50, 3, 4)
+
+
+def test_read_velocity_segy_cdp_line_and_single_trace(tmp_path):
+    from seisproc_amd.utils.segy import write_segy
+    from seisproc_amd.utils.velocity_io import read_velocity_segy
+    t = 0.002 * np.arange(30)
+    v = 1800.0 + 500.0 * t[None, :] + 20.0 * np.arange(5)[:, None]
+    f = tmp_path / "line.segy"
+    write_segy(f, v[::-1], 0.002, cdp=np.array([50, 40, 30, 20, 10]), delay_ms=0)
+    m = read_velocity_segy(f)
+    assert m.metadata["lateral_axes"] == "cdp" and m.metadata["cdp_range"] == (10, 50)
+    np.testing.assert_array_equal(m.x_axis, [10, 20, 30, 40, 50])
+    np.testing.assert_allclose(m.data, v.T, rtol=0, atol=1e-3)  # sorted by CDP
+    one = tmp_path / "one.sgy"
+    write_segy(one, v[:1], 0.002, delay_ms=8)
+    m1 = read_velocity_segy(one)
+    assert m1.data.ndim == 1 and abs(m1.z_axis[0] - 0.008) < 1e-12
+    np.testing.assert_allclose(m1.data, v[0], rtol=0, atol=1e-3)
+
+
+def test_preview_and_info_to_model(tmp_path):
+    from seisproc_amd.utils.segy import write_segy
+    from seisproc_amd.utils.velocity_io import preview_velocity_file, velocity_info_to_model
+    t, il, xl, v = _cube()
+    f = tmp_path / "vel.sgy"
+    write_segy(f, v, 0.004, inline=il, crossline=xl)
+    info = preview_velocity_file(f)
+    assert info.is_valid and info.n_locations == 12 and info.n_time_samples == 50
+    assert info.inline_range == (101, 103) and info.xline_range == (7, 10) and info.sample_interval == 0.004
+    assert info.velocity_range[0] == pytest.approx(v.min(), abs=1e-3)
+    assert velocity_info_to_model(info).data.shape == (50, 3, 4)
+    txt = tmp_path / "tv.txt"
+    txt.write_text("# t v\n0 1500\n1000 2500\n2000 3100\n")
+    i2 = preview_velocity_file(txt)
+    assert i2.is_valid and i2.raw_data["format"] == "tv" and i2.sample_interval == 1000.0
+    m = velocity_info_to_model(i2, time_unit="ms", velocity_unit="km/s")
+    np.testing.assert_allclose(m.z_axis, [0.0, 1.0, 2.0])
+    np.testing.assert_allclose(m.data, [1.5e6, 2.5e6, 3.1e6])
+    bad = preview_velocity_file(tmp_path / "missing.txt")
+    assert not bad.is_valid and "not found" in bad.error_message
+    with pytest.raises(ValueError, match="Invalid"):
+        velocity_info_to_model(bad)
+
+
+def test_models_from_picks():
+    from seisproc_amd.utils.anisotropy_io import create_anisotropy_from_picks
+    from seisproc_amd.utils.velocity_io import create_velocity_from_picks
+    m = create_velocity_from_picks([(1.0, 2500.0), (0.0, 1500.0), (2.0, 3000.0)], dt=0.5)
+    np.testing.assert_allclose(m.z_axis, [0.0, 0.5, 1.0, 1.5, 2.0])
+    np.testing.assert_allclose(m.data, [1500, 2000, 2500, 2750, 3000])
+    assert m.metadata["n_picks"] == 3
+    m2 = create_velocity_from_picks([(0.5, 2000.0)], dt=0.25, t_max=1.0)
+    np.testing.assert_allclose(m2.data, [2000.0] * 5)
+    with pytest.raises(ValueError):
+        create_velocity_from_picks([])
+    a = create_anisotropy_from_picks([(0.0, 0.1, 0.05), (1.0, 0.2, 0.05)], dz=0.5)
+    np.testing.assert_allclose(a.epsilon, [0.1, 0.15, 0.2], rtol=1e-6)
+    np.testing.assert_allclose(a.eta, (np.array([0.1, 0.15, 0.2]) - 0.05) / 1.1, rtol=1e-6)
