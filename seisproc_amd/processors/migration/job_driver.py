@@ -356,15 +356,22 @@ class MigrationJobRunner:
                 cur.traces.record_stream(cs)
                 cur.geometry.record_stream(cs)
             fn = self.migrate_fn or self._migrate_hip
+            ev = None
+            if self.device.type == "cuda":
+                ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                ev[0].record(torch.cuda.current_stream(self.device))
             img, fold = fn(p, cur.traces, cur.geometry)
             if self.stack:
                 st_img += img
                 st_fold += fold
+            if ev is not None:
+                ev[1].record(torch.cuda.current_stream(self.device))
             # the next bin's gather and copy overlap this bin's migration
             nxt = stage(k + 1) if k + 1 < len(todo) else None
             if self.device.type == "cuda":
                 torch.cuda.synchronize(self.device)
             t_mig = time.perf_counter() - t0
+            gpu_s = ev[0].elapsed_time(ev[1]) / 1e3 if ev is not None else None
             extra = {"bin_name": name, "input_traces": int(idx.size), "max_aperture_m": p.aperture_m,
                      "max_angle_deg": p.max_angle_deg, "trace_dtype": self.trace_dtype}
             t1 = time.perf_counter()
@@ -374,7 +381,7 @@ class MigrationJobRunner:
             if self.checkpoint:
                 self._save_state(key, completed, st_img if self.stack else None, st_fold if self.stack else None)
             res.completed.append(name)
-            res.timings[name] = {"stage_host_s": cur.host_seconds, "migrate_s": t_mig,
+            res.timings[name] = {"stage_host_s": cur.host_seconds, "migrate_s": t_mig, "gpu_s": gpu_s,
                                  "write_s": time.perf_counter() - t1, "traces": int(idx.size)}
             del img, fold, cur
             if progress_callback:
