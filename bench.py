@@ -569,13 +569,6 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(1)
-    if args.workload == "lsm3d":
-        _bench_lsm(args, torch)
-        return
-    if args.workload == "job":
-        _bench_job(args, torch)
-        return
-    t_start = time.perf_counter()
     if args.copy_probe_gib > 0:
         n = int(args.copy_probe_gib * (1 << 30)) // 4
         a = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -583,6 +576,13 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if args.workload == "lsm3d":
+        _bench_lsm(args, torch)
+        return
+    if args.workload == "job":
+        _bench_job(args, torch)
+        return
+    t_start = time.perf_counter()
     s, cfg, forms = _workload(args, torch)
     if args.forms:
         forms = [f for f in args.forms.split(",") if f]
