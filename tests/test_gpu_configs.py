@@ -119,9 +119,9 @@ def test_compensated_accumulation_beats_plain_fp32():
     # into one image (100 calls, up to 6400 contributions per sample; plain
     # FP32 accumulation error grows with the calls, the compensated pair's
     # does not: what remains is the FP32 arithmetic of each contribution).
-    # Compensated (image +
-    # comp pair, TwoSum joins) against plain FP32 accumulation, both against
-    # the FP64 oracle, on the samples whose fold agrees with the oracle's (an
+    # Compensated (image + comp pair, TwoSum joins) against plain FP32
+    # accumulation, both against the FP64 oracle, on the samples whose fold
+    # agrees with the oracle's (an
     # aperture-edge flip changes a sample by a whole contribution) at 0.1 s
     # and later (earlier, the oracle's 1e-6 m epsilons exceed 1e-8 of the
     # weights).
