@@ -12,8 +12,6 @@ Contents
                   the C code's loops and index handling).
   ref_torch.py    FP32 torch-CPU restatement of the reference tile loop; timed as
                   the CPU baseline.
-  synth.py        seeded synthetic survey generators (Ricker events at exact DSR
-                  times), following tests/fixtures/synthetic_prestack.py:28-106.
 
 Parity status (how each oracle function is pinned; see tests/test_oracle_*.py):
   forward      known answers restated from the reference tests (DSR times

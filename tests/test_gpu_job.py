@@ -59,4 +59,6 @@ def test_job_bf16_traces(tmp_path):
                              checkpoint=False).run(src)
     ref, _ = oracle.forward(p, s.traces.double().numpy(), *s.numpy_geometry(), s.vel.double().numpy())
     got = res.stack_image.double().cpu().numpy()
-    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 5e-3
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print(f"[job bf16] rel_rms vs FP64 oracle: {err:.3e}")
+    assert err < 1e-3

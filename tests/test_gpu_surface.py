@@ -64,7 +64,9 @@ def test_engine_rms_velocity_and_bf16():
     ref = ref.transpose(2, 0, 1)
     assert np.linalg.norm(img - ref) / np.linalg.norm(ref) < 1e-4
     b16, _ = eng.migrate_bin_full(**kw, velocity_model=s.vel, trace_dtype="bf16")
-    assert np.linalg.norm(b16 - ref) / np.linalg.norm(ref) < 5e-3
+    e16 = np.linalg.norm(b16 - ref) / np.linalg.norm(ref)
+    print(f"[engine bf16] rel_rms vs FP64 oracle: {e16:.3e}")
+    assert e16 < 1e-3
     # 1D RMS model object
     vm = VelocityModel(data=np.array([1800.0, 2300.0], np.float32), z_axis=np.array([0.0, 1.0]))
     img1, _ = eng.migrate_bin_full(**kw, velocity_model=vm)
