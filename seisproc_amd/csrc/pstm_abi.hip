@@ -170,6 +170,7 @@ void fill_tile_args(const pstm_params_t* p, const PlanLayout& L, const void* ws,
   t.aa_scale_over_dt = (float)(p->aa_scale / p->dt);
   t.aa_max = (float)p->aa_max;
   t.th_max = (float)(p->max_angle_deg * M_PI / 180.0);
+  t.tan_taper = (float)tan(p->max_angle_deg * M_PI / 180.0 * (1.0 - p->taper_width));
   t.off_min = (float)p->min_offset_m;
   t.off_max = p->max_offset_m > 0 ? (float)p->max_offset_m : 0.f;
   t.aa_pad = (p->flags & PSTM_FLAG_AA) ? (int)ceil(p->aa_max) + 2 : 0;

@@ -103,8 +103,9 @@ template <int NC, bool AA, int OCC, bool FX>
 constexpr int window_cap_elems() {
   return (FX && !AA) ? sample_cap_floats<NC, AA, OCC>() / 2 : sample_cap_elems<NC, AA, OCC>();
 }
-// The antialias forward at 3 workgroups per CU stores 16-byte window
-// elements (FP64 double integral and FP32 single integral side by side).
+// The antialias forward at 3 or more workgroups per CU takes 16 bytes per
+// window element (FP64 double integral, and FP32 single integral in an
+// 8-byte slot of a second array).
 template <int NC, int OCC = 3>
 constexpr int aa_fast_cap_elems() {
   return sample_cap_floats<NC, true, OCC>() / 4;
@@ -181,6 +182,7 @@ struct TileArgs {
   float taper_width;
   float aa_scale_over_dt, aa_max;
   float th_max;             // max_angle in radians (taper)
+  float tan_taper;          // tan(th_max (1 - taper_width)): the angle taper's flat zone ends there
   float off_min, off_max;   // offset window (taper; off_max <= 0: none)
   int32_t aa_pad;           // staged zero padding on each side (samples)
   int32_t flags, weight_mode, vel_mode;

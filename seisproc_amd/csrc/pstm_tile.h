@@ -201,7 +201,7 @@ __device__ __forceinline__ void two_sum_add(float& s, float& c, float y) {
 // 0 beyond (selects: a zero-width zone never reaches the division's NaN).
 __device__ __forceinline__ float edge_taper(float x, float xmax, float tw) {
   const float s = xmax * (1.0f - tw);
-  const float c = 0.5f * (1.0f + __cosf(3.14159265358979f * (x - s) / (xmax - s)));
+  const float c = 0.5f * (1.0f + __cosf(3.14159265358979f * __fdividef(x - s, xmax - s)));
   return x > xmax ? 0.f : (x > s ? c : 1.f);
 }
 
@@ -214,9 +214,17 @@ __device__ __forceinline__ float edge_taper(float x, float xmax, float tw) {
 __device__ __forceinline__ float aperture_taper(float hs2, float hr2, float z, const TileArgs& a) {
   float ap = a.aperture;
   if (a.flags & PSTM_FLAG_TIME_APERTURE) ap = fminf(ap, z * a.tan_th);
-  const float hs = sqrtf(hs2), hr = sqrtf(hr2), tw = a.taper_width;
-  return edge_taper(hs, ap, tw) * edge_taper(hr, ap, tw) * edge_taper(atan2f(hs, z), a.th_max, tw) *
-         edge_taper(atan2f(hr, z), a.th_max, tw);
+  const float hs = __builtin_amdgcn_sqrtf(hs2), hr = __builtin_amdgcn_sqrtf(hr2), tw = a.taper_width;
+  // each factor is 1 in its flat zone (distance up to ap (1 - tw), angle up
+  // to th_max (1 - tw), i.e. h <= z tan_taper): a wave evaluates the cosine
+  // (and the angle) only when one of its lanes lies beyond
+  const float sd = ap * (1.0f - tw), tz = z * a.tan_taper;
+  float w = 1.f;
+  if (hs > sd) w *= edge_taper(hs, ap, tw);
+  if (hr > sd) w *= edge_taper(hr, ap, tw);
+  if (hs > tz) w *= edge_taper(atan2f(hs, z), a.th_max, tw);
+  if (hr > tz) w *= edge_taper(atan2f(hr, z), a.th_max, tw);
+  return w;
 }
 
 // Offset window taper of one trace (aperture.py:184-224): near taper
@@ -279,9 +287,13 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   constexpr bool PREFETCH = MODE != MODE_ADJ && !AA && !VTI && !GEN && (OCC <= 3 || (V1 && OCC <= 4));
   static_assert(NC == 8 || (NC == 32 && MODE == MODE_ADJ), "columns per tile: 8 (32 for the adjoint)");
   static_assert(!FIXED || MODE == MODE_ADJ, "fixed-point windows are an adjoint form");
-  // ILV (antialias forward at 3 workgroups per CU): each window element is 16
-  // bytes, the FP64 double integral then the FP32 single integral, so that
-  // the three taps of a contribution need one address each
+  // ILV (antialias forward at 3 or more workgroups per CU): the FP64 double
+  // integral A at an 8-byte stride and the FP32 single integral C in a second
+  // array, also at an 8-byte stride, a fixed 8 CAPE bytes after A, so that a
+  // tap reads both through one address (C through the instruction's
+  // immediate offset).  Neighbouring samples of A fill distinct bank pairs;
+  // C's 8-byte stride costs a 2-way conflict on its 4-byte reads (16-byte
+  // elements holding A and C side by side conflicted 2-way on A, 4-way on C).
   constexpr bool ILV = AA && MODE == MODE_FWD && OCC >= 3;
   // PAIR (plain forward at 4 or more workgroups per CU): each window element
   // is a sample and its forward difference (a[i], a[i+1] - a[i]) in FP32, so
@@ -294,8 +306,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   static_assert(!CAN_SEG || GEN, "segmenting forms are general forms");
   constexpr int CAPF = sample_cap_floats<NC, AA, OCC>();         // staged-window floats
   constexpr int CAPE = ILV ? CAPF / 4 : PAIR ? CAPF / 2 : window_cap_elems<NC, AA, OCC, FX>();  // staged-window elements
-  constexpr int SA = ILV ? 2 : 1;                                // element stride in doubles (Wa)
-  constexpr int SC = ILV ? 4 : 1;                                // element stride in floats (Wc)
+  constexpr int SA = 1;                                          // element stride in doubles (Wa)
+  constexpr int SC = ILV ? 2 : 1;                                // element stride in floats (Wc)
   // every staged element, FP64 A and FP32 C, lies inside samp[CAPF]
   static_assert(!AA || (ILV ? 4 * CAPE <= CAPF : 3 * CAPE <= CAPF), "antialias window layout exceeds the LDS buffer");
   constexpr int MB = batch_of(OCC);  // traces per sub-batch
@@ -307,7 +319,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   [[maybe_unused]] float* const Wf = samp;
   [[maybe_unused]] float2* const Wp = reinterpret_cast<float2*>(samp);  // PAIR elements
   [[maybe_unused]] double* const Wa = reinterpret_cast<double*>(samp);
-  [[maybe_unused]] float* const Wc = ILV ? samp + 2 : samp + 2 * CAPE;  // ILV: stride 4 floats
+  [[maybe_unused]] float* const Wc = samp + 2 * CAPE;  // ILV: stride 2 floats
   [[maybe_unused]] long long* const Wl = reinterpret_cast<long long*>(samp);  // adjoint, fixed point
 
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -718,7 +730,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
             float wd = prs * prs;
             if constexpr (GEN) {
               if (wmode != PSTM_WEIGHT_FULL) wd = wmode == PSTM_WEIGHT_NONE ? 1.f : prs;
-              if (use_taper) wd *= aperture_taper(h2.x, h2.y, sqrtf(tqs / qd[c]), a);
+              if (use_taper)  // z = v tau / 2 = sqrt(tqs / qd)
+                wd *= aperture_taper(h2.x, h2.y, __builtin_amdgcn_sqrtf(tqs * __builtin_amdgcn_rcpf(qd[c])), a);
             } else if constexpr (WM == PSTM_WEIGHT_SPREADING) {
               wd = prs;
             }
@@ -1124,8 +1137,11 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               const float hw = S.hwave[w];
               if (hs_min < hw && hr_min < hw) {
                 const float ta = fmaxf(S.tqa[w], 1e-12f), tbq = S.tqb[w] + 1e-12f;
-                const float lo = sqrtf(ta + hs_min * S.qmin[w] * ecorr) + sqrtf(ta + hr_min * S.qmin[w] * ecorr);
-                const float hi = sqrtf(tbq + hs_max * S.qmax[w] * ecorr_hi) + sqrtf(tbq + hr_max * S.qmax[w] * ecorr_hi);
+                // hardware square roots (1 ulp, far inside the window margin)
+                const float lo = __builtin_amdgcn_sqrtf(ta + hs_min * S.qmin[w] * ecorr) +
+                                 __builtin_amdgcn_sqrtf(ta + hr_min * S.qmin[w] * ecorr);
+                const float hi = __builtin_amdgcn_sqrtf(tbq + hs_max * S.qmax[w] * ecorr_hi) +
+                                 __builtin_amdgcn_sqrtf(tbq + hr_max * S.qmax[w] * ecorr_hi);
                 tlo = fminf(tlo, lo);
                 thi = fmaxf(thi, hi);
                 // wave w never reads past either end of the record (the staged
