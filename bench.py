@@ -55,6 +55,13 @@ HBM_PEAK_GBS = 8000.0
 # per second (one FMA = 2 FLOP): 78.65 T lane-op/s, i.e. one wave64 FP32
 # instruction per 2 cycles per SIMD at 2.4 GHz on 1024 SIMDs.
 VALU_PEAK_TLANE = 157.3 / 2.0
+# That figure counts packed FP32 (two values per lane per instruction), which
+# this build never emits.  Unpacked FP32 VALU instructions issue at 16 lanes
+# per clock per SIMD -- the lane rate of the published FP64 vector peak, 78.6
+# TFLOP/s -- so the VALU ceiling of these kernels is half the published FP32
+# figure: 39.32 T lane-op/s (1024 SIMDs x 16 lanes x 2.4 GHz).  The issue-rate
+# probe (seisproc_amd/csrc/valu_rates.hip) measures v_fma_f32 streams there.
+VALU_UNPACKED_TLANE = 1024 * 16 * 2.4e9 / 1e12
 # LDS: 256 CUs x 128 bytes per clock x 2.4 GHz.
 LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12
 N_SIMD = 1024
@@ -127,6 +134,8 @@ def roofline(contrib: float, kernel_s: float, bytes_per_contribution: int, cnt):
     write = c["WRITE_SIZE_KiB"] * 1024 * rec["calibration"]["write_factor"] * scale
     lds_bytes = (c["SQ_INSTS_LDS_LOAD_BANDWIDTH"] + c["SQ_INSTS_LDS_STORE_BANDWIDTH"]) * 64 * scale
     r.update(achieved=round(ach, 2), frac=round(ach / VALU_PEAK_TLANE, 4), traffic=round(fetch + write))
+    r["unpacked_fp32"] = {"peak": round(VALU_UNPACKED_TLANE, 2), "frac": round(ach / VALU_UNPACKED_TLANE, 4),
+                          "note": "the ceiling without packed FP32: 16 lanes per clock per SIMD"}
     r["valu_per_contribution"] = round(c["SQ_INSTS_VALU"] * 64 / rec["contributions"], 2)
     r["trans_per_contribution"] = round(c["SQ_INSTS_VALU_TRANS_F32"] * 64 / rec["contributions"], 2)
     r["issue_cycles_per_valu"] = round((c["SQ_BUSY_CYCLES"] / N_SE) / (c["SQ_INSTS_VALU"] / N_SIMD), 2)

@@ -20,6 +20,10 @@ LIB = HERE / "libpstm_hip.so"
 SOURCES = ["pstm_abi.hip", "pstm_plan.hip", "pstm_fwd_f32.hip", "pstm_fwd_bf16.hip", "pstm_fwd_f16.hip",
            "pstm_adj.hip", "pstm_grad.hip", "pstm_static.hip"]
 ARCH = os.environ.get("PSTM_OFFLOAD_ARCH", "gfx950")
+# Standalone probe (not part of the library): VALU issue rates of the
+# instruction kinds the tile kernels use, ./seisproc_amd/valu_rates
+PROBE_SRC = "valu_rates.hip"
+PROBE = HERE / "valu_rates"
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -62,7 +66,21 @@ def build(verbose: bool = False, extra: list[str] | None = None) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_probe(verbose)
     return LIB
+
+
+def build_probe(verbose: bool = False) -> Path:
+    """Compile the VALU issue-rate probe (an executable, same flags)."""
+    tmp = PROBE.with_name(PROBE.name + ".tmp")
+    cmd = [_hipcc(), *HIPCC_FLAGS, str(CSRC / PROBE_SRC), "-o", str(tmp)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {PROBE_SRC}:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, PROBE)
+    if verbose:
+        print(f"built {PROBE}")
+    return PROBE
 
 
 if __name__ == "__main__":

@@ -63,6 +63,9 @@ def test_counter_roofline_from_a_summary_record():
     assert r["valu_per_contribution"] == pytest.approx(19.2)
     assert r["issue_cycles_per_valu"] == pytest.approx((4.0e11 / 32) / (3.0e12 / 1024), rel=1e-3)
     assert r["hbm_algorithmic"]["frac"] == pytest.approx(1.0e13 * 4 / 5.0 / 1e9 / 8000.0, rel=1e-3)
+    # unpacked FP32 ceiling: half the published (packed) FP32 figure
+    assert r["unpacked_fp32"]["peak"] == pytest.approx(r["peak"] / 2, rel=2e-3)
+    assert r["unpacked_fp32"]["frac"] == pytest.approx(r["achieved"] / r["unpacked_fp32"]["peak"], rel=1e-3)
     stale = bench.roofline(1.0e13, 5.0, 4, (None, "no summary"))
     assert stale["achieved"] is None and stale["traffic"] is None and stale["counters"]["note"] == "no summary"
 
