@@ -152,6 +152,11 @@ __device__ __forceinline__ unsigned long long to_fixed64(double d) {
   const unsigned long long h = (unsigned long long)(unsigned int)(int)hi;
   return (h << 32) | (unsigned long long)(unsigned int)lo;
 }
+// Window element of trace sample i in a bank-skewed adjoint window: slot
+// i + floor(i / 32) (one spare slot before every 32nd sample).
+__device__ __forceinline__ unsigned long long* skewed(unsigned long long* base, int i) {
+  return base + (i + (i >> 5));
+}
 __device__ __forceinline__ double from_fixed64(long long v) {
   return fma((double)(int)(v >> 32), 0x1p32, (double)(unsigned int)(v & 0xffffffffll));
 }
@@ -282,6 +287,10 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   // without the mask, clamp and fold instructions (their fold is counted once
   // per wave); the rest take the masked loop.  Both give the same sums.
   constexpr bool FULLPATH = MODE == MODE_FWD && !GEN;
+  // Adjoint counterpart (plain fixed-point forms): traces that stay inside the
+  // record in a wave AND pass every mask at every time and column of it
+  // scatter without the mask and without the per-lane skip branch.
+  constexpr bool ADJ_UNM = MODE == MODE_ADJ && !GEN && !AA;
   // Software prefetch of the next trace's distances (registers permitting:
   // the plain forward and gradient forms).
   constexpr bool PREFETCH = MODE != MODE_ADJ && !AA && !VTI && !GEN && (OCC <= 3 || (V1 && OCC <= 4));
@@ -342,19 +351,26 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   // below 1/2 sample per sample, while a wave spans 128 times (all four waves
   // interleaved over 256 times leave more lanes outside the masks: 1.6 %
   // slower; contiguous times put many lanes' taps on one sample: 7 % slower).
-  const int k = MODE == MODE_ADJ ? tb * kTileTau + (wave >> 1) * 128 + lane * 2 + (wave & 1) : tb * kTileTau + tid;
-  const bool kvalid = k < a.n_tau;
-  const int kk = kvalid ? k : a.n_tau - 1;
+  //
+  // The time blocks are aligned to the END of the output times: block tb
+  // starts at tb * kTileTau - kshift, so the one ragged block (with idle
+  // lanes) is the shallowest, where the angle mask admits the fewest traces,
+  // not the deepest and heaviest one.
+  const int kshift = a.n_tblocks * kTileTau - a.n_tau;  // 0 .. kTileTau - 1
+  const int kb0 = tb * kTileTau - kshift;               // first time of the block (may be < 0)
+  const int k = MODE == MODE_ADJ ? kb0 + (wave >> 1) * 128 + lane * 2 + (wave & 1) : kb0 + tid;
+  const bool kvalid = k >= 0;  // k < n_tau always
+  const int kk = kvalid ? k : 0;
   const float tau = a.tau0 + (float)kk * a.dtau;
   const float tq = 0.25f * tau * tau;
   const float idt2 = a.inv_dt * a.inv_dt;
   const float tqs = fmaxf(tq, 1e-12f) * idt2;
-  [[maybe_unused]] const float vfloor = 1e-12f * idt2;
+  [[maybe_unused]] const float tqs2 = tqs * tqs;  // VTI: constant term of the anelliptic numerator
   const int wmode = GEN ? a.weight_mode : WM;
   [[maybe_unused]] const float aa_scale = AA ? a.aa_scale_over_dt / a.inv_dt : 0.f;
 
   float qd[NC], hl2[NC], cw[NC];
-  [[maybe_unused]] float k1[VTI ? NC : 1], m2[VTI ? NC : 1];
+  [[maybe_unused]] float k1[VTI ? NC : 1], c1[VTI ? NC : 1];  // 1 + 2 eta, 2 (1 + eta) tqs
   [[maybe_unused]] float sv[MODE == MODE_GRAD ? NC : 1];  // 1/v
   [[maybe_unused]] float qa[AA ? NC : 1];                 // qd * aa_scale
   // eta extremes of the tile (VTI), each clamped to 0 from its side: they
@@ -392,7 +408,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
     if constexpr (VTI) {
       const float e = ok ? a.eta[vi] : 0.f;
       k1[c] = 1.0f + 2.0f * e;
-      m2[c] = -2.0f * e;
+      c1[c] = (2.0f + 2.0f * e) * tqs;
       emax_l = fmaxf(emax_l, e);
       emin_l = fminf(emin_l, e);
     }
@@ -407,7 +423,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   for (int c = 0; c < NC; ++c) {
     const float hc = wave_max(hl2[c]);
     if (lane == 0) P.hcol[wave][c] = hc;
-    if constexpr (FULLPATH) {
+    if constexpr (FULLPATH || ADJ_UNM) {
       const float hn = wave_min(hl2[c]);
       if (lane == 0) P.hcolmin[wave][c] = hn;
     }
@@ -419,10 +435,9 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
       S.hwave[wave] = hw;
       S.qmin[wave] = qn;
       S.qmax[wave] = qx;
-      const int ka = MODE == MODE_ADJ ? min(tb * kTileTau + (wave >> 1) * 128 + (wave & 1), a.n_tau - 1)
-                                      : min(tb * kTileTau + wave * 64, a.n_tau - 1);
-      const int kb = MODE == MODE_ADJ ? min(tb * kTileTau + (wave >> 1) * 128 + 126 + (wave & 1), a.n_tau - 1)
-                                      : min(tb * kTileTau + wave * 64 + 63, a.n_tau - 1);
+      // first and last valid time of the wave (a wave with none has no reach)
+      const int ka = max(MODE == MODE_ADJ ? kb0 + (wave >> 1) * 128 + (wave & 1) : kb0 + wave * 64, 0);
+      const int kb = max(MODE == MODE_ADJ ? kb0 + (wave >> 1) * 128 + 126 + (wave & 1) : kb0 + wave * 64 + 63, 0);
       const float ta = a.tau0 + (float)ka * a.dtau, tbv = a.tau0 + (float)kb * a.dtau;
       S.tqa[wave] = 0.25f * ta * ta;
       S.tqb[wave] = 0.25f * tbv * tbv;
@@ -658,13 +673,14 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
 #pragma unroll
           for (int w = 0; w < kWaves; ++w) {
             bits |= hm < P.hcol[w][c] ? (1 << w) : 0;
-            if constexpr (FULLPATH) nfb |= hm < P.hcolmin[w][c] ? 0 : (1 << w);
+            if constexpr (FULLPATH || ADJ_UNM) nfb |= hm < P.hcolmin[w][c] ? 0 : (1 << w);
           }
           bits = group_or<NC>(bits);
-          if constexpr (FULLPATH) nfb = group_or<NC>(nfb);  // a wave some column does not fully pass
+          if constexpr (FULLPATH || ADJ_UNM) nfb = group_or<NC>(nfb);  // a wave some column does not fully pass
           // bits 4-7: passes every mask and stays in the record; bits 8-11: passes
           // every mask but some time reads past the record (clamped)
           if constexpr (FULLPATH) bits |= ((S.q_nc[qs + b] & ~nfb & 0xf) << 4) | ((~S.q_nc[qs + b] & ~nfb & 0xf) << 8);
+          else if constexpr (ADJ_UNM) bits |= ((S.q_nc[qs + b] & 0xf) << 4) | ((S.q_nc[qs + b] & ~nfb & 0xf) << 8);
           else if constexpr (MODE == MODE_ADJ && !AA) bits |= (S.q_nc[qs + b] & 0xf) << 4;
           if (c == NC - 1) P.act[b] = bits;
         }
@@ -682,10 +698,11 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           constexpr bool SEG = decltype(seg_tag)::value;
           // kind 1 (FULL): the trace's times stay inside the record in this
           // wave (no clamp); in the forward it also passes every mask.  Kind 2
-          // (forward): it passes every mask but may clamp.  Kind 0: general.
+          // (forward): it passes every mask but may clamp.  Kind 3 (adjoint):
+          // inside the record and passes every mask.  Kind 0: general.
           constexpr int KIND = decltype(full_tag)::value;
-          constexpr bool FULL = KIND == 1;
-          constexpr bool UNMASKED = KIND != 0 && MODE == MODE_FWD;
+          constexpr bool FULL = KIND == 1 || KIND == 3;
+          constexpr bool UNMASKED = (KIND != 0 && MODE == MODE_FWD) || KIND == 3;
           constexpr bool NARROW = decltype(narrow_tag)::value;
           // adjoint, clamping traces: the contributions read past the record
           // end all land on samples n-2 and n-1; they are summed in a register
@@ -715,12 +732,15 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
             const float rs = __builtin_amdgcn_rsqf(xs), rr = __builtin_amdgcn_rsqf(xr);
             float u_raw;
             if constexpr (VTI) {
+              // anelliptic leg, p = h^2 qd (samples^2):
+              //   t^2 = tqs + p - 2 eta p^2 / (tqs + (1 + 2 eta) p) = N / D,
+              //   N = p^2 + 2 (1 + eta) tqs p + tqs^2,  D = tqs + (1 + 2 eta) p,
+              // both sums of positive terms for 1 + 2 eta > 0 (no cancellation,
+              // N >= tqs^2 > 0); t = N rsq(N D): one transcendental per leg
               const float ps = h2.x * qd[c], pr = h2.y * qd[c];
+              const float ns = fmaf(ps, ps + c1[c], tqs2), nr = fmaf(pr, pr + c1[c], tqs2);
               const float ds = fmaf(k1[c], ps, tqs), dr = fmaf(k1[c], pr, tqs);
-              const float vs = fmaf(m2[c] * ps, ps * __builtin_amdgcn_rcpf(ds), tqs + ps);
-              const float vr = fmaf(m2[c] * pr, pr * __builtin_amdgcn_rcpf(dr), tqs + pr);
-              u_raw = __builtin_amdgcn_sqrtf(fmaxf(vs, vfloor)) + __builtin_amdgcn_sqrtf(fmaxf(vr, vfloor)) -
-                      a.t0_over_dt;
+              u_raw = fmaf(ns, __builtin_amdgcn_rsqf(ns * ds), fmaf(nr, __builtin_amdgcn_rsqf(nr * dr), -a.t0_over_dt));
             } else {
               u_raw = fmaf(xs, rs, fmaf(xr, rr, -a.t0_over_dt));
             }
@@ -810,19 +830,21 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
                 gsc = fmaf(-gsc, cl, gsc);
               }
               // lanes without a contribution skip the atomics: their positions
-              // may lie outside the staged window, and clamped lanes share one
-              if (gsc != 0.f) {
+              // may lie outside the staged window, and clamped lanes share one.
+              // Kind 3 needs no skip: every lane of the wave passes the masks
+              // and reads inside the window (a zero tap adds nothing).
+              if (KIND == 3 || gsc != 0.f) {
                 if constexpr (SKEW && NARROW) {
                   // gsc < 2^30 in scaled units; each tap rounded toward zero
-                  const int pos = i0 + (i0 >> 5);
+                  unsigned long long* const q = skewed(Lb, i0);
                   const float t1 = gsc * f, t0 = gsc - t1;
-                  atomicAdd(&Lb[pos], (unsigned long long)(long long)(int)t0);
-                  atomicAdd(&Lb[pos + 1], (unsigned long long)(long long)(int)t1);
+                  atomicAdd(q, (unsigned long long)(long long)(int)t0);
+                  atomicAdd(q + 1, (unsigned long long)(long long)(int)t1);
                 } else if constexpr (SKEW) {
-                  const int pos = i0 + (i0 >> 5);
+                  unsigned long long* const q = skewed(Lb, i0);
                   const double g = (double)gsc * fx_scale, gf = g * (double)f;
-                  atomicAdd(&Lb[pos], to_fixed64(g - gf));
-                  atomicAdd(&Lb[pos + 1], to_fixed64(gf));
+                  atomicAdd(q, to_fixed64(g - gf));
+                  atomicAdd(q + 1, to_fixed64(gf));
                 } else if constexpr (FX) {
                   const double g = (double)gsc * fx_scale, gf = g * (double)f;
                   if constexpr (AA) {
@@ -946,13 +968,17 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
         }
         if (CAN_SEG && seg) {
         } else if constexpr (MODE == MODE_ADJ && !AA) {
-          // traces inside the record in this wave, then those that clamp
+          // traces inside the record in this wave that pass every mask (ADJ_UNM),
+          // the other traces inside the record, then those that clamp
           tfull = __ballot(lane < nb && ((P.act[lane] >> (4 + wave)) & 1));
+          const uint64_t tunm = ADJ_UNM ? __ballot(lane < nb && ((P.act[lane] >> (8 + wave)) & 1)) : 0;
           if (!SKEW || narrow) {
-            walk(tfull, std::integral_constant<int, 1>{}, std::true_type{}, std::false_type{});
+            if constexpr (ADJ_UNM) walk(tunm, std::integral_constant<int, 3>{}, std::true_type{}, std::false_type{});
+            walk(tfull & ~tunm, std::integral_constant<int, 1>{}, std::true_type{}, std::false_type{});
             walk(todo & ~tfull, std::integral_constant<int, 0>{}, std::true_type{}, std::false_type{});
           } else {
-            walk(tfull, std::integral_constant<int, 1>{}, std::false_type{}, std::false_type{});
+            if constexpr (ADJ_UNM) walk(tunm, std::integral_constant<int, 3>{}, std::false_type{}, std::false_type{});
+            walk(tfull & ~tunm, std::integral_constant<int, 1>{}, std::false_type{}, std::false_type{});
             walk(todo & ~tfull, std::integral_constant<int, 0>{}, std::false_type{}, std::false_type{});
           }
         } else {

@@ -47,16 +47,32 @@ def _need_cuda(t: torch.Tensor, name: str, dtype=None, shape=None) -> torch.Tens
     return t
 
 
-def _check_eta(eta: Optional[torch.Tensor], vel: torch.Tensor) -> None:
+# Largest squared isotropic leg time (samples^2) of a VTI call: the kernels
+# evaluate each anelliptic leg as t = N rsq(N D) with N <= (tqs + p)^2 and
+# D <= max(1, 1 + 2 eta) (tqs + p), which stays inside the FP32 range while
+# max(1, 1 + 2 eta)^(1/3) (tqs + p) < 6.9e12.
+VTI_MAX_LEG2 = float(2 ** 42)
+
+
+def _check_eta(eta: Optional[torch.Tensor], vel: torch.Tensor, p: "PstmParams") -> None:
     """The kernels take eta as given: the anelliptic time and its window
-    bounds need 1 + 2 eta > 0 at every sample (one device reduction and a
-    synchronisation per call)."""
+    bounds need 1 + 2 eta > 0 at every sample, and the leg times must stay
+    within the FP32 range of the anelliptic product (one device reduction and
+    a synchronisation per call)."""
     if eta is None:
         raise ValueError("VTI migration needs eta")
     _need_cuda(eta, "eta", torch.float32, tuple(vel.shape))
-    lo = float(eta.min()) if eta.numel() else 0.0
+    if not eta.numel():
+        return
+    lo, hi, vmin = (float(x) for x in torch.stack([eta.min(), eta.max(), vel.min()]).cpu())
     if not lo > -0.5:
         raise ValueError(f"eta must be > -0.5 everywhere (1 + 2 eta > 0), min is {lo}")
+    if vmin > 0.0:
+        tau_max = p.tau0 + (p.n_tau - 1) * p.dtau
+        leg2 = (tau_max / (2.0 * p.dt)) ** 2 + (p.aperture_m / (vmin * p.dt)) ** 2
+        if leg2 * max(1.0, 1.0 + 2.0 * hi) ** (1.0 / 3.0) >= VTI_MAX_LEG2:
+            raise ValueError(f"VTI leg times up to {leg2 ** 0.5:.3g} samples (aperture {p.aperture_m} m, "
+                             f"v_min {vmin} m/s, dt {p.dt} s) exceed the anelliptic range of 2^21 samples")
 
 
 class PstmPlan:
@@ -157,7 +173,7 @@ class PstmPlan:
         if scale is not None:
             _need_cuda(scale, "scale", torch.float32, (p.n_traces,))
         if cp.flags & FLAG_VTI:
-            _check_eta(eta, vel)
+            _check_eta(eta, vel, cp)
         if image is None:
             image = torch.zeros(self.image_shape, dtype=torch.float32, device=self.device)
         _need_cuda(image, "image", torch.float32, self.image_shape)
@@ -191,7 +207,7 @@ class PstmPlan:
         vm = self._vel_mode(vel)
         cp = self._call_params(flags, PSTM_F32, vm)
         if cp.flags & FLAG_VTI:
-            _check_eta(eta, vel)
+            _check_eta(eta, vel, cp)
         if out is None:
             out = torch.zeros((p.n_traces, p.n_samples), dtype=torch.float32, device=self.device)
         _need_cuda(out, "out", torch.float32, (p.n_traces, p.n_samples))
