@@ -157,6 +157,13 @@ __device__ __forceinline__ unsigned long long to_fixed64(double d) {
 __device__ __forceinline__ unsigned long long* skewed(unsigned long long* base, int i) {
   return base + (i + (i >> 5));
 }
+// The same for |d| < 2^51 in two instructions: adding 1.5 * 2^52 leaves
+// round(d) + 2^51 in the low 52 bits of the mantissa at a fixed exponent, so
+// the sum's bit pattern minus that of 1.5 * 2^52 (low word 0, high word
+// 0x43380000) is round(d), rounded to nearest even as rint does.
+__device__ __forceinline__ unsigned long long to_fixed64_small(double d) {
+  return (unsigned long long)__double_as_longlong(d + 0x1.8p52) - 0x4338000000000000ull;
+}
 __device__ __forceinline__ double from_fixed64(long long v) {
   return fma((double)(int)(v >> 32), 0x1p32, (double)(unsigned int)(v & 0xffffffffll));
 }
@@ -368,11 +375,11 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   [[maybe_unused]] const float tqs2 = tqs * tqs;  // VTI: constant term of the anelliptic numerator
   const int wmode = GEN ? a.weight_mode : WM;
   [[maybe_unused]] const float aa_scale = AA ? a.aa_scale_over_dt / a.inv_dt : 0.f;
+  [[maybe_unused]] const float sil = a.d_il * aa_scale, sxl = a.d_xl * aa_scale;  // antialias: scaled column steps
 
-  float qd[NC], hl2[NC], cw[NC];
+  float qd[NC], hl2[NC];
   [[maybe_unused]] float k1[VTI ? NC : 1], c1[VTI ? NC : 1];  // 1 + 2 eta, 2 (1 + eta) tqs
   [[maybe_unused]] float sv[MODE == MODE_GRAD ? NC : 1];  // 1/v
-  [[maybe_unused]] float qa[AA ? NC : 1];                 // qd * aa_scale
   // eta extremes of the tile (VTI), each clamped to 0 from its side: they
   // bound the anelliptic leg time between sqrt(t0^2 + p / (1 + 2 max(eta, 0)))
   // and sqrt(t0^2 + p / (1 + 2 min(eta, 0))), p = h^2 / v^2 (1 + 2 eta > 0)
@@ -394,12 +401,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
     const float vv = ok ? v : 1.f;
     const float q = 1.0f / (vv * vv);
     qd[c] = q * idt2;
-    if (wmode == PSTM_WEIGHT_FULL) cw[c] = (tq * q) * (idt2 * idt2);
-    else if (wmode == PSTM_WEIGHT_SPREADING) cw[c] = q * idt2;
-    else if (wmode == PSTM_WEIGHT_OBLIQUITY) cw[c] = tq * idt2;
-    else cw[c] = 1.f;
     if constexpr (MODE == MODE_GRAD) sv[c] = 1.0f / vv;
-    if constexpr (AA) qa[c] = qd[c] * aa_scale;
     const float z = 0.5f * vv * tau;
     float ap = a.aperture;
     if (a.flags & PSTM_FLAG_TIME_APERTURE) ap = fminf(ap, z * a.tan_th);
@@ -418,6 +420,16 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
     }
     hmax_l = fmaxf(hmax_l, hl2[c]);
   }
+  // Weight constant of column c: tau^2/4 q / dt^4 (FULL), q / dt^2
+  // (SPREADING), tau^2/4 / dt^2 (OBLIQUITY) or 1 (NONE), formed from qd when
+  // needed (at the flushes) rather than held per column in registers.
+  const float cwt = tq * idt2;
+  auto cw = [&](int c) -> float {
+    if (wmode == PSTM_WEIGHT_FULL) return cwt * qd[c];
+    if (wmode == PSTM_WEIGHT_SPREADING) return qd[c];
+    if (wmode == PSTM_WEIGHT_OBLIQUITY) return cwt;
+    return 1.f;
+  };
   // per-wave column reach (max and min over the wave's times)
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
@@ -477,11 +489,11 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const int il = tx * TX + c / TY, xl = ty * TY + c % TY;
-      m_in[c] = (kvalid && ((cvalid >> c) & 1u)) ? a.image_in[((int64_t)il * a.n_xl + xl) * a.n_tau + k] * cw[c]
+      m_in[c] = (kvalid && ((cvalid >> c) & 1u)) ? a.image_in[((int64_t)il * a.n_xl + xl) * a.n_tau + k] * cw(c)
                                                  : 0.f;
     }
   }
-  [[maybe_unused]] double fx_scale = 0.0, fx_inv = 0.0;
+  [[maybe_unused]] double fx_inv = 0.0;
   bool narrow = false;  // SKEW: taps in FP32 at 2^(30 - e) (else FP64 at 2^(47 - e))
   if constexpr (FX) {
     // bound of one contribution: |m cw| times the largest weight power
@@ -510,16 +522,12 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
     (void)frexpf(G, &e);  // G < 2^e
     // FP32 taps rounded at 2^-30 G when the per-lane bounds span at most a
     // factor kNarrowSpan; else FP64 taps rounded at 2^-47 G
+    // (the image values carry the scale, a power of two: exact in FP32)
     narrow = SKEW && G <= kNarrowSpan * Gmin;
-    if (narrow) {
-      // taps |w m| 2^(30 - e) < 2^30: the image values carry the scale
+    const int sh = narrow ? 30 - e : 47 - e;  // taps |w m| 2^sh < 2^30 or 2^47
 #pragma unroll
-      for (int c = 0; c < NC; ++c) m_in[c] = ldexpf(m_in[c], 30 - e);
-      fx_inv = ldexp(1.0, e - 30);
-    } else {
-      fx_scale = ldexp(1.0, 47 - e);
-      fx_inv = ldexp(1.0, e - 47);
-    }
+    for (int c = 0; c < NC; ++c) m_in[c] = ldexpf(m_in[c], sh);
+    fx_inv = ldexp(1.0, -sh);
   }
 
   const bool use_taper = GEN ? (a.flags & PSTM_FLAG_TAPER) != 0 : false;
@@ -550,13 +558,13 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
       if (a.flags & PSTM_FLAG_KAHAN) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-          two_sum_add(acc[c], cmp[c], accb[c] * cw[c]);
+          two_sum_add(acc[c], cmp[c], accb[c] * cw(c));
           accb[c] = 0.f;
         }
       } else {
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-          acc[c] = fmaf(accb[c], cw[c], acc[c]);
+          acc[c] = fmaf(accb[c], cw(c), acc[c]);
           accb[c] = 0.f;
         }
       }
@@ -668,7 +676,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           const float hm = fmaxf(hs, hr);
           P.h2[b][c] = make_float2(hs, hr);
           P.hm[b][c] = hm;
-          if constexpr (AA) P.g[b][c] = make_float4(dxs * a.d_il, dys * a.d_xl, dxr * a.d_il, dyr * a.d_xl);
+          // the triangle half-width's scale is folded in: L = qd (|g.x rs + g.z rr| + |g.y rs + g.w rr|)
+          if constexpr (AA) P.g[b][c] = make_float4(dxs * sil, dys * sxl, dxr * sil, dyr * sxl);
           int bits = 0, nfb = 0;
 #pragma unroll
           for (int w = 0; w < kWaves; ++w) {
@@ -759,7 +768,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
             if constexpr (AA) {
               const float4 g = P.g[b][c];
               const float ga = fmaf(g.x, rs, g.z * rr), gb = fmaf(g.y, rs, g.w * rr);
-              L = __builtin_amdgcn_fmed3f(qa[c] * (fabsf(ga) + fabsf(gb)), 1.f, a.aa_max);
+              L = __builtin_amdgcn_fmed3f(qd[c] * (fabsf(ga) + fabsf(gb)), 1.f, a.aa_max);
               iL2 = __builtin_amdgcn_rcpf(L * L);
             }
             // SEG: only sample indices of this segment count; the other
@@ -841,26 +850,27 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
                   atomicAdd(q, (unsigned long long)(long long)(int)t0);
                   atomicAdd(q + 1, (unsigned long long)(long long)(int)t1);
                 } else if constexpr (SKEW) {
+                  // |g| < 2^47 (the workgroup's scale): two-instruction rounding
                   unsigned long long* const q = skewed(Lb, i0);
-                  const double g = (double)gsc * fx_scale, gf = g * (double)f;
-                  atomicAdd(q, to_fixed64(g - gf));
-                  atomicAdd(q + 1, to_fixed64(gf));
+                  const double g = (double)gsc, gf = g * (double)f;
+                  atomicAdd(q, to_fixed64_small(g - gf));
+                  atomicAdd(q + 1, to_fixed64_small(gf));
                 } else if constexpr (FX) {
-                  const double g = (double)gsc * fx_scale, gf = g * (double)f;
+                  const double g = (double)gsc, gf = g * (double)f;
                   if constexpr (AA) {
                     const float um = us - L, up = us + L;
                     const int im = (int)floorf(um), ip = (int)up;
                     const double fm = (double)(um - (float)im), fp = (double)(up - (float)ip);
                     const double gc = 2.0 * g * (double)iL2, ge = -g * (double)iL2, gcf = gc * (double)f;
-                    atomicAdd(&Lb[i0], to_fixed64(gc - gcf));
-                    atomicAdd(&Lb[i0 + 1], to_fixed64(gcf));
-                    atomicAdd(&Lb[im], to_fixed64(ge - ge * fm));
-                    atomicAdd(&Lb[im + 1], to_fixed64(ge * fm));
-                    atomicAdd(&Lb[ip], to_fixed64(ge - ge * fp));
-                    atomicAdd(&Lb[ip + 1], to_fixed64(ge * fp));
+                    atomicAdd(&Lb[i0], to_fixed64_small(gc - gcf));
+                    atomicAdd(&Lb[i0 + 1], to_fixed64_small(gcf));
+                    atomicAdd(&Lb[im], to_fixed64_small(ge - ge * fm));
+                    atomicAdd(&Lb[im + 1], to_fixed64_small(ge * fm));
+                    atomicAdd(&Lb[ip], to_fixed64_small(ge - ge * fp));
+                    atomicAdd(&Lb[ip + 1], to_fixed64_small(ge * fp));
                   } else {
-                    atomicAdd(&Lb[i0], to_fixed64(g - gf));
-                    atomicAdd(&Lb[i0 + 1], to_fixed64(gf));
+                    atomicAdd(&Lb[i0], to_fixed64_small(g - gf));
+                    atomicAdd(&Lb[i0 + 1], to_fixed64_small(gf));
                   }
                 } else if constexpr (AA) {
                   const float um = us - L, up = us + L;
@@ -888,10 +898,9 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
                 const float fc = __builtin_amdgcn_fractf(a.umax);
                 const float t1 = tot * fc, t0 = tot - t1;
                 if constexpr (FX) {
-                  const double sc = NARROW ? 1.0 : fx_scale;  // narrow: already in scaled units
-                  const int pos = SKEW ? ic + (ic >> 5) : ic;
-                  atomicAdd(&Lb[pos], to_fixed64((double)t0 * sc));
-                  atomicAdd(&Lb[pos + 1], to_fixed64((double)t1 * sc));
+                  const int pos = SKEW ? ic + (ic >> 5) : ic;  // taps already in scaled units
+                  atomicAdd(&Lb[pos], to_fixed64((double)t0));
+                  atomicAdd(&Lb[pos + 1], to_fixed64((double)t1));
                 } else {
                   atomicAdd(&Fb[ic], t0);
                   atomicAdd(&Fb[ic + 1], t1);
