@@ -79,7 +79,9 @@ struct PairShared {
   float hm[MB][NC];             // max of the two
   int32_t act[MB];              // bit w: the trace reaches some time of wave w; bit 4 + w: it stays
                                 // inside the record in wave w and (forward) passes every time and
-                                // column of it; bit 8 + w (forward): passes them all but clamps
+                                // column of it; bit 8 + w (forward): passes them all but clamps;
+                                // (plain adjoint) bit 8 + w: inside the record and passes them all,
+                                // bit 12 + w: passes them all but clamps
   float hcol[kWaves][NC];       // per wave and column: max hlim^2 over the wave's times
   float hcolmin[kWaves][NC];    // per wave and column: min hlim^2 over the wave's times
   float4 g[AA ? MB : 1][AA ? NC : 1];  // antialias: scaled (s - x, r - x) components

@@ -330,6 +330,11 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   __shared__ TileShared<MB> S;
   __shared__ PairShared<NC, AA, MB> P;
   __shared__ __align__(16) float samp[CAPF];
+  // Plain adjoint: per trace of the sub-batch, the sum of its contributions
+  // that read past the record end (all land on samples n-2 and n-1; split
+  // between them at the flush).  MB floats fit in the LDS budget's slack.
+  constexpr bool CLAMPSUM = MODE == MODE_ADJ && !AA;
+  [[maybe_unused]] __shared__ float clampS[CLAMPSUM ? MB : 1];
   // Staged windows: FP32 samples, or with antialias the FP64 double integral
   // (Wa) and the FP32 single integral (Wc, forward only).
   [[maybe_unused]] float* const Wf = samp;
@@ -591,6 +596,9 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           const int w0 = S.q_w0[e], len = S.q_len[e], off = S.b_off[b];
           if constexpr (MODE == MODE_ADJ) {
             const int span = SKEW ? skew_len(len) : len;
+            if constexpr (CLAMPSUM) {
+              if (lane == 0) clampS[b] = 0.f;
+            }
             for (int s = lane; s < span; s += 64) {
               if constexpr (FX) Wl[off + s] = 0;
               else if constexpr (AA) Wa[off + s] = 0.0;
@@ -689,7 +697,9 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           // bits 4-7: passes every mask and stays in the record; bits 8-11: passes
           // every mask but some time reads past the record (clamped)
           if constexpr (FULLPATH) bits |= ((S.q_nc[qs + b] & ~nfb & 0xf) << 4) | ((~S.q_nc[qs + b] & ~nfb & 0xf) << 8);
-          else if constexpr (ADJ_UNM) bits |= ((S.q_nc[qs + b] & 0xf) << 4) | ((S.q_nc[qs + b] & ~nfb & 0xf) << 8);
+          else if constexpr (ADJ_UNM)
+            bits |= ((S.q_nc[qs + b] & 0xf) << 4) | ((S.q_nc[qs + b] & ~nfb & 0xf) << 8) |
+                    ((~S.q_nc[qs + b] & ~nfb & 0xf) << 12);
           else if constexpr (MODE == MODE_ADJ && !AA) bits |= (S.q_nc[qs + b] & 0xf) << 4;
           if (c == NC - 1) P.act[b] = bits;
         }
@@ -708,10 +718,11 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           // kind 1 (FULL): the trace's times stay inside the record in this
           // wave (no clamp); in the forward it also passes every mask.  Kind 2
           // (forward): it passes every mask but may clamp.  Kind 3 (adjoint):
-          // inside the record and passes every mask.  Kind 0: general.
+          // inside the record and passes every mask; kind 4 (adjoint): passes
+          // every mask but may clamp.  Kind 0: general.
           constexpr int KIND = decltype(full_tag)::value;
           constexpr bool FULL = KIND == 1 || KIND == 3;
-          constexpr bool UNMASKED = (KIND != 0 && MODE == MODE_FWD) || KIND == 3;
+          constexpr bool UNMASKED = (KIND != 0 && MODE == MODE_FWD) || KIND == 3 || KIND == 4;
           constexpr bool NARROW = decltype(narrow_tag)::value;
           // adjoint, clamping traces: the contributions read past the record
           // end all land on samples n-2 and n-1; they are summed in a register
@@ -724,6 +735,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           [[maybe_unused]] double* const Ab = Wa + SA * base;
           [[maybe_unused]] float* const Cb = Wc + SC * base;
           [[maybe_unused]] unsigned long long* const Lb = reinterpret_cast<unsigned long long*>(Wl + base);
+          // antialias: the next column's slopes are read one column ahead
+          [[maybe_unused]] float4 gnext = AA ? P.g[b][0] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
           for (int c = 0; c < NC; ++c) {
             const float2 h2 = h2v[c];
@@ -766,7 +779,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
             }
             [[maybe_unused]] float L = 1.f, iL2 = 1.f;
             if constexpr (AA) {
-              const float4 g = P.g[b][c];
+              const float4 g = gnext;
+              if (c + 1 < NC) gnext = P.g[b][c + 1];
               const float ga = fmaf(g.x, rs, g.z * rr), gb = fmaf(g.y, rs, g.w * rr);
               L = __builtin_amdgcn_fmed3f(qd[c] * (fabsf(ga) + fabsf(gb)), 1.f, a.aa_max);
               iL2 = __builtin_amdgcn_rcpf(L * L);
@@ -794,8 +808,13 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
                 const float flm = floorf(um);
                 const int im = (int)flm, ip = (int)up;
                 const float fm = um - flm, fp = __builtin_amdgcn_fractf(up);
-                const double y2 = fma(2.0, Ab[SA * i0], -Ab[SA * im]) - Ab[SA * ip];
-                const float yc = fmaf(-2.f * f, Cb[SC * i0], fmaf(fm, Cb[SC * im], fp * Cb[SC * ip]));
+                const double a0 = Ab[SA * i0], am = Ab[SA * im], ap = Ab[SA * ip];
+                const float c0 = Cb[SC * i0], cm = Cb[SC * im], cp = Cb[SC * ip];
+                // the six reads are issued together: one LDS round trip per column
+                // (left to itself the scheduler spaces them over three)
+                __builtin_amdgcn_sched_barrier(0);
+                const double y2 = fma(2.0, a0, -am) - ap;
+                const float yc = fmaf(-2.f * f, c0, fmaf(fm, cm, fp * cp));
                 amp = ((float)y2 + yc) * iL2;
               } else if constexpr (PAIR) {
                 const float2 e = Pb[i0];
@@ -891,21 +910,14 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
             }
           }
           if constexpr (MODE == MODE_ADJ && !AA && !FULL) {
+            // row sums (16 lanes each), added by four lanes to the trace's clamp sum
             if (__ballot(cacc != 0.f) != 0) {
-              const float tot = wave_scan(cacc, lane);  // lane 63: the wave's sum
-              if (lane == 63 && tot != 0.f) {
-                const int ic = (int)a.umax;  // clamped position: sample n-2, fraction 0.999
-                const float fc = __builtin_amdgcn_fractf(a.umax);
-                const float t1 = tot * fc, t0 = tot - t1;
-                if constexpr (FX) {
-                  const int pos = SKEW ? ic + (ic >> 5) : ic;  // taps already in scaled units
-                  atomicAdd(&Lb[pos], to_fixed64((double)t0));
-                  atomicAdd(&Lb[pos + 1], to_fixed64((double)t1));
-                } else {
-                  atomicAdd(&Fb[ic], t0);
-                  atomicAdd(&Fb[ic + 1], t1);
-                }
-              }
+              float rsum = cacc;
+              rsum += dpp_move<kDppRowShr1>(rsum);
+              rsum += dpp_move<kDppRowShr2>(rsum);
+              rsum += dpp_move<kDppRowShr4>(rsum);
+              rsum += dpp_move<kDppRowShr8>(rsum);
+              if ((lane & 15) == 15 && rsum != 0.f) atomicAdd(&clampS[b], rsum);
             }
           }
         };
@@ -981,14 +993,21 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
           // the other traces inside the record, then those that clamp
           tfull = __ballot(lane < nb && ((P.act[lane] >> (4 + wave)) & 1));
           const uint64_t tunm = ADJ_UNM ? __ballot(lane < nb && ((P.act[lane] >> (8 + wave)) & 1)) : 0;
+          const uint64_t tunc = ADJ_UNM ? __ballot(lane < nb && ((P.act[lane] >> (12 + wave)) & 1)) : 0;
           if (!SKEW || narrow) {
-            if constexpr (ADJ_UNM) walk(tunm, std::integral_constant<int, 3>{}, std::true_type{}, std::false_type{});
+            if constexpr (ADJ_UNM) {
+              walk(tunm, std::integral_constant<int, 3>{}, std::true_type{}, std::false_type{});
+              walk(tunc, std::integral_constant<int, 4>{}, std::true_type{}, std::false_type{});
+            }
             walk(tfull & ~tunm, std::integral_constant<int, 1>{}, std::true_type{}, std::false_type{});
-            walk(todo & ~tfull, std::integral_constant<int, 0>{}, std::true_type{}, std::false_type{});
+            walk(todo & ~tfull & ~tunc, std::integral_constant<int, 0>{}, std::true_type{}, std::false_type{});
           } else {
-            if constexpr (ADJ_UNM) walk(tunm, std::integral_constant<int, 3>{}, std::false_type{}, std::false_type{});
+            if constexpr (ADJ_UNM) {
+              walk(tunm, std::integral_constant<int, 3>{}, std::false_type{}, std::false_type{});
+              walk(tunc, std::integral_constant<int, 4>{}, std::false_type{}, std::false_type{});
+            }
             walk(tfull & ~tunm, std::integral_constant<int, 1>{}, std::false_type{}, std::false_type{});
-            walk(todo & ~tfull, std::integral_constant<int, 0>{}, std::false_type{}, std::false_type{});
+            walk(todo & ~tfull & ~tunc, std::integral_constant<int, 0>{}, std::false_type{}, std::false_type{});
           }
         } else {
           walk(todo & ~tfull, std::integral_constant<int, 0>{}, std::true_type{}, std::false_type{});
@@ -1023,6 +1042,15 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
             // transpose of the forward's per-trace offset taper
             const float ow = use_taper ? offset_taper(a.offs[S.q_j[e]], a) : 1.f;
             [[maybe_unused]] const long long* const Lw = Wl + S.b_base[b];
+            // contributions that read past the record end: sample n-2 takes
+            // (1 - fc) of their sum, n-1 takes fc (fc = the fraction of n - 1.001)
+            [[maybe_unused]] float cl_lo = 0.f, cl_hi = 0.f;
+            if constexpr (CLAMPSUM) {
+              const float fc = __builtin_amdgcn_fractf(a.umax);
+              const float cs = clampS[b] * (FX ? (float)fx_inv : 1.f);
+              cl_hi = cs * fc;
+              cl_lo = cs - cl_hi;
+            }
             for (int s = lane; s < len; s += 64) {
               const int dst = w0 + s;
               float v;
@@ -1033,6 +1061,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               } else if constexpr (FX && AA) v = (float)(Wa[off + s] * fx_inv);
               else if constexpr (FX) v = (float)(from_fixed64(Wl[off + s]) * fx_inv);
               else v = AA ? (float)Wa[off + s] : Wf[off + s];
+              if constexpr (CLAMPSUM) v += dst == a.n_samples - 2 ? cl_lo : (dst == a.n_samples - 1 ? cl_hi : 0.f);
               if (dst >= 0 && dst < a.n_samples && v != 0.f) atomicAdd(&row[dst], v * ow);
             }
           }

@@ -240,3 +240,17 @@ def test_eta_at_or_below_minus_half_is_rejected():
     bad[0, 0, 0] = -0.5
     with pytest.raises(ValueError, match="eta"):
         plan.forward(s.traces.to(DEV), s.vel.to(DEV), eta=bad.to(DEV))
+
+
+def test_vti_leg_times_beyond_the_anelliptic_range_are_rejected():
+    # each anelliptic leg is evaluated as N rsq(N D), N ~ t^4 and D ~ t^2 in
+    # samples: leg times of ~2^21 samples and more would leave the FP32 range,
+    # so the call is refused (here: a velocity of 1e-3 m/s over 4000 m)
+    s = S.vti_largest_3d(seed=6, n_classes=2, class_min=10, class_max=20, n_samples=101, n_img=4, d_img=12.5,
+                         n_diff=2)
+    plan = _plan(s)
+    slow = torch.full_like(s.vel, 1e-3)
+    with pytest.raises(ValueError, match="anelliptic range"):
+        plan.forward(s.traces.to(DEV), slow.to(DEV), eta=s.eta.to(DEV))
+    with pytest.raises(ValueError, match="anelliptic range"):
+        plan.adjoint(torch.zeros(plan.image_shape, device=DEV), slow.to(DEV), eta=s.eta.to(DEV))
