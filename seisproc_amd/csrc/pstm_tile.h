@@ -332,8 +332,10 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
   __shared__ __align__(16) float samp[CAPF];
   // Plain adjoint: per trace of the sub-batch, the sum of its contributions
   // that read past the record end (all land on samples n-2 and n-1; split
-  // between them at the flush).  MB floats fit in the LDS budget's slack.
-  constexpr bool CLAMPSUM = MODE == MODE_ADJ && !AA;
+  // between them at the flush).  The MB floats fit in the LDS budget's slack
+  // of the forms at 5 or more workgroups per CU; the others scatter a wave sum
+  // instead (their LDS shares end within a granule of the per-CU capacity).
+  constexpr bool CLAMPSUM = MODE == MODE_ADJ && !AA && OCC >= 5;
   [[maybe_unused]] __shared__ float clampS[CLAMPSUM ? MB : 1];
   // Staged windows: FP32 samples, or with antialias the FP64 double integral
   // (Wa) and the FP32 single integral (Wc, forward only).
@@ -909,7 +911,7 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               }
             }
           }
-          if constexpr (MODE == MODE_ADJ && !AA && !FULL) {
+          if constexpr (MODE == MODE_ADJ && !AA && !FULL && CLAMPSUM) {
             // row sums (16 lanes each), added by four lanes to the trace's clamp sum
             if (__ballot(cacc != 0.f) != 0) {
               float rsum = cacc;
@@ -918,6 +920,23 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               rsum += dpp_move<kDppRowShr4>(rsum);
               rsum += dpp_move<kDppRowShr8>(rsum);
               if ((lane & 15) == 15 && rsum != 0.f) atomicAdd(&clampS[b], rsum);
+            }
+          } else if constexpr (MODE == MODE_ADJ && !AA && !FULL) {
+            if (__ballot(cacc != 0.f) != 0) {
+              const float tot = wave_scan(cacc, lane);  // lane 63: the wave's sum
+              if (lane == 63 && tot != 0.f) {
+                const int ic = (int)a.umax;  // clamped position: sample n-2, fraction 0.999
+                const float fc = __builtin_amdgcn_fractf(a.umax);
+                const float t1 = tot * fc, t0 = tot - t1;
+                if constexpr (FX) {
+                  const int pos = SKEW ? ic + (ic >> 5) : ic;  // taps already in scaled units
+                  atomicAdd(&Lb[pos], to_fixed64((double)t0));
+                  atomicAdd(&Lb[pos + 1], to_fixed64((double)t1));
+                } else {
+                  atomicAdd(&Fb[ic], t0);
+                  atomicAdd(&Fb[ic + 1], t1);
+                }
+              }
             }
           }
         };
