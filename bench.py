@@ -84,10 +84,12 @@ def source_sha() -> str:
     return h.hexdigest()[:16]
 
 
-def load_counters(workload: str, form: str, classes):
+def load_counters(workload: str, form: str, classes, any_classes: bool = False):
     """Per-call counter totals of this workload and trace form from the JSON
     block of a profiles/*.md summary stamped with this build's source digest.
-    Returns (record, path) or (None, reason)."""
+    any_classes: also accept a summary of the same workload on another number
+    of offset classes (the caller scales its totals by contributions; the
+    record keeps its own "classes").  Returns (record, path) or (None, reason)."""
     sha = source_sha()
     found = []
     for f in sorted(PROFILES.glob("*.md")):
@@ -99,8 +101,10 @@ def load_counters(workload: str, form: str, classes):
                 continue
             if rec.get("kind") != "pstm-counters":
                 continue
-            if rec.get("workload") == workload and rec.get("form") == form and rec.get("classes") == classes:
+            if rec.get("workload") == workload and rec.get("form") == form and (
+                    any_classes or rec.get("classes") == classes):
                 found.append((rec, f))
+    found.sort(key=lambda rf: rf[0].get("classes") != classes)  # the same class count first
     for rec, f in found:
         if rec.get("source_sha") == sha:
             return rec, str(f.relative_to(ROOT))
@@ -147,6 +151,9 @@ def roofline(contrib: float, kernel_s: float, bytes_per_contribution: int, cnt):
                 "insts_per_contribution": round(c["SQ_INSTS_LDS"] * 64 / rec["contributions"], 2),
                 "bank_conflict_cycle_frac": round(c["SQ_LDS_BANK_CONFLICT"] / max(c["SQ_LDS_IDX_ACTIVE"], 1), 4)}
     r["counters"] = {"source": src, "source_sha": rec["source_sha"]}
+    if abs(scale - 1.0) > 1e-9:
+        r["counters"]["scaled_by_contributions"] = round(scale, 4)
+        r["counters"]["profiled_classes"] = rec.get("classes")
     return r
 
 
@@ -263,10 +270,18 @@ def _error_columns(s, images, folds, n_cols, torch):
         g = np.concatenate(gots[k])
         bar = 1e-4 if k == "fp32" else 1e-3
         err = float(np.linalg.norm(g - ref) / np.linalg.norm(ref))
-        flips = float(np.abs(np.concatenate(gfolds[k]) - rfold).sum())
-        out[k] = {"rel_rms": err, "rel_max": float(np.abs(g - ref).max() / np.abs(ref).max()),
+        gf = np.concatenate(gfolds[k])
+        flips = float(np.abs(gf - rfold).sum())
+        # an aperture- or angle-edge flip (FP32 and FP64 masks deciding a
+        # boundary tie differently) moves its sample by a whole contribution:
+        # flips have their own bar, the image bar applies to the samples whose
+        # fold agrees with the oracle's
+        agree = gf == rfold
+        err_agree = float(np.linalg.norm((g - ref)[agree]) / np.linalg.norm(ref[agree]))
+        out[k] = {"rel_rms": err, "rel_rms_fold_agree": err_agree,
+                  "rel_max": float(np.abs(g - ref).max() / np.abs(ref).max()),
                   "fold_flips": flips, "contributions_checked": float(rfold.sum()), "error_bar": bar,
-                  "error_ok": bool(err <= bar and flips <= 1e-4 * rfold.sum() + 1)}
+                  "error_ok": bool(err_agree <= bar and flips <= 1e-4 * rfold.sum() + 1)}
     return out, f"FP64 oracle on {len(picks)} image columns {picks} (all traces in aperture)"
 
 
@@ -669,7 +684,8 @@ def main():
         print(f"[bench] errors vs FP64 oracle: {err}", file=sys.stderr, flush=True)
         for k, e in err.items():
             if not e["error_ok"]:
-                print(f"[bench] WARNING: the {k} form misses its error bar ({e['rel_rms']:.3e} > {e['error_bar']})",
+                print(f"[bench] WARNING: the {k} form misses its error bar (fold-agreeing {e['rel_rms_fold_agree']:.3e}, "
+                      f"all {e['rel_rms']:.3e} against {e['error_bar']}; {e['fold_flips']:.0f} fold flips)",
                       file=sys.stderr, flush=True)
     del images, folds
 
@@ -683,9 +699,12 @@ def main():
     taps = 6 if aa else 2
     sbytes = 4 if main_form == "fp32" else 2
     B = taps * sbytes
-    classes = cfg.get("offset_classes", cfg.get("classes"))
+    classes = cfg.get("offset_classes", cfg.get("classes", 1))  # single-offset: one class
+    # the largest VTI workload is profiled on 12 of its classes (a counter pass
+    # over all 60 takes minutes per counter set); its totals scale by contributions
     rl = roofline(m["contributions"], m["kernel_s"], B,
-                  load_counters(cfg["workload"] + ("-vt" if args.vt else ""), main_form, classes))
+                  load_counters(cfg["workload"] + ("-vt" if args.vt else ""), main_form, classes,
+                                any_classes=cfg["workload"] == "pstm3d-vti-largest"))
     rl["kernel"] = "k_tile forward (one pstm_forward call: 16 offset-band dispatches)"
     cpu = None
     if not args.no_cpu_baseline:
