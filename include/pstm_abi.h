@@ -135,8 +135,8 @@ int pstm_quantize_traces(const float* traces, int64_t n_traces, int32_t n_sample
  *   eta:         FP32, layout p->vel_mode, required with PSTM_FLAG_VTI, else ignored;
  *                every value must satisfy 1 + 2 eta > 0, and the isotropic leg times
  *                must stay below 2^21 samples: ((tau_max / 2)^2 + (aperture / v_min)^2)
- *                / dt^2 * max(1, 1 + 2 eta_max)^(1/3) < 2^42 (neither is checked on the
- *                device; the Python layer checks both)
+ *                / dt^2 * ((1 + e) (1 + 2 e))^(1/3) < 2^42, e = max(eta_max, 0) (neither
+ *                is checked on the device; the Python layer checks both)
  *   image:       FP32 [il][xl][tau], image += sum of contributions
  *   image_comp:  FP32 [il][xl][tau] running compensation (PSTM_FLAG_KAHAN), or NULL
  *                (then the compensation is folded into image)

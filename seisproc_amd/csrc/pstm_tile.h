@@ -153,9 +153,19 @@ __device__ __forceinline__ unsigned long long to_fixed64(double d) {
   return (h << 32) | (unsigned long long)(unsigned int)lo;
 }
 // Window element of trace sample i in a bank-skewed adjoint window: slot
-// i + floor(i / 32) (one spare slot before every 32nd sample).
-__device__ __forceinline__ unsigned long long* skewed(unsigned long long* base, int i) {
-  return base + (i + (i >> 5));
+// i + floor(i / 32) (one spare slot before every 32nd sample).  Its LDS byte
+// address base + 8 i + 8 floor(i / 32) is formed with two shift-adds (left to
+// itself the compiler spends two shifts and a three-operand add).
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+__device__ __forceinline__ lds_u64* skewed(unsigned long long* base, int i) {
+  const uint32_t b = (uint32_t)(uintptr_t)(lds_u64*)base;
+  uint32_t t, a;
+  asm("v_lshl_add_u32 %0, %1, 3, %2" : "=v"(t) : "v"(i), "v"(b));
+  asm("v_lshl_add_u32 %0, %1, 3, %2" : "=v"(a) : "v"(i >> 5), "v"(t));
+  return (lds_u64*)(uintptr_t)a;
+}
+__device__ __forceinline__ void lds_add(lds_u64* q, unsigned long long v) {
+  __hip_atomic_fetch_add(q, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 // The same for |d| < 2^51 in two instructions: adding 1.5 * 2^52 leaves
 // round(d) + 2^51 in the low 52 bits of the mantissa at a fixed exponent, so
@@ -816,8 +826,8 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
                 // (left to itself the scheduler spaces them over three)
                 __builtin_amdgcn_sched_barrier(0);
                 const double y2 = fma(2.0, a0, -am) - ap;
-                const float yc = fmaf(-2.f * f, c0, fmaf(fm, cm, fp * cp));
-                amp = ((float)y2 + yc) * iL2;
+                // the FP32 corrections -2 f C(i0) + fm C(im) + fp C(ip) chained onto it
+                amp = fmaf(-2.f * f, c0, fmaf(fm, cm, fmaf(fp, cp, (float)y2))) * iL2;
               } else if constexpr (PAIR) {
                 const float2 e = Pb[i0];
                 amp = fmaf(f, e.y, e.x);
@@ -866,16 +876,16 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               if (KIND == 3 || gsc != 0.f) {
                 if constexpr (SKEW && NARROW) {
                   // gsc < 2^30 in scaled units; each tap rounded toward zero
-                  unsigned long long* const q = skewed(Lb, i0);
+                  lds_u64* const q = skewed(Lb, i0);
                   const float t1 = gsc * f, t0 = gsc - t1;
-                  atomicAdd(q, (unsigned long long)(long long)(int)t0);
-                  atomicAdd(q + 1, (unsigned long long)(long long)(int)t1);
+                  lds_add(q, (unsigned long long)(long long)(int)t0);
+                  lds_add(q + 1, (unsigned long long)(long long)(int)t1);
                 } else if constexpr (SKEW) {
                   // |g| < 2^47 (the workgroup's scale): two-instruction rounding
-                  unsigned long long* const q = skewed(Lb, i0);
+                  lds_u64* const q = skewed(Lb, i0);
                   const double g = (double)gsc, gf = g * (double)f;
-                  atomicAdd(q, to_fixed64_small(g - gf));
-                  atomicAdd(q + 1, to_fixed64_small(gf));
+                  lds_add(q, to_fixed64_small(g - gf));
+                  lds_add(q + 1, to_fixed64_small(gf));
                 } else if constexpr (FX) {
                   const double g = (double)gsc, gf = g * (double)f;
                   if constexpr (AA) {

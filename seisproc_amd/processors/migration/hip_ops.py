@@ -47,10 +47,11 @@ def _need_cuda(t: torch.Tensor, name: str, dtype=None, shape=None) -> torch.Tens
     return t
 
 
-# Largest squared isotropic leg time (samples^2) of a VTI call: the kernels
-# evaluate each anelliptic leg as t = N rsq(N D) with N <= (tqs + p)^2 and
-# D <= max(1, 1 + 2 eta) (tqs + p), which stays inside the FP32 range while
-# max(1, 1 + 2 eta)^(1/3) (tqs + p) < 6.9e12.
+# Largest squared isotropic leg time (samples^2) of a VTI call, scaled: the
+# kernels evaluate each anelliptic leg as t = N rsq(N D) with
+# N <= (1 + e) (tqs + p)^2 and D <= (1 + 2 e) (tqs + p), e = max(eta, 0), so
+# N D < 2^126 (a quarter of the FP32 maximum) while
+# ((1 + e) (1 + 2 e))^(1/3) (tqs + p) < 2^42.
 VTI_MAX_LEG2 = float(2 ** 42)
 
 
@@ -68,11 +69,21 @@ def _check_eta(eta: Optional[torch.Tensor], vel: torch.Tensor, p: "PstmParams") 
     if not lo > -0.5:
         raise ValueError(f"eta must be > -0.5 everywhere (1 + 2 eta > 0), min is {lo}")
     if vmin > 0.0:
-        tau_max = p.tau0 + (p.n_tau - 1) * p.dtau
-        leg2 = (tau_max / (2.0 * p.dt)) ** 2 + (p.aperture_m / (vmin * p.dt)) ** 2
-        if leg2 * max(1.0, 1.0 + 2.0 * hi) ** (1.0 / 3.0) >= VTI_MAX_LEG2:
+        leg2 = vti_leg2_bound(p, vmin, hi)
+        if leg2 >= VTI_MAX_LEG2:
             raise ValueError(f"VTI leg times up to {leg2 ** 0.5:.3g} samples (aperture {p.aperture_m} m, "
                              f"v_min {vmin} m/s, dt {p.dt} s) exceed the anelliptic range of 2^21 samples")
+
+
+def vti_leg2_bound(p: "PstmParams", vmin: float, eta_max: float) -> float:
+    """((1 + e) (1 + 2 e))^(1/3), e = max(eta_max, 0), times the largest
+    squared isotropic leg time (samples^2) of a call: latest output time, a
+    leg at the aperture, the smallest velocity.  Below VTI_MAX_LEG2 the
+    kernels' anelliptic product N D stays below 2^126."""
+    tau_max = p.tau0 + (p.n_tau - 1) * p.dtau
+    leg2 = (tau_max / (2.0 * p.dt)) ** 2 + (p.aperture_m / (vmin * p.dt)) ** 2
+    e = max(eta_max, 0.0)
+    return leg2 * ((1.0 + e) * (1.0 + 2.0 * e)) ** (1.0 / 3.0)
 
 
 class PstmPlan:

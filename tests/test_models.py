@@ -235,3 +235,39 @@ def test_optimized_migrator_factory_has_no_cpu_path():
     with pytest.raises(RuntimeError, match="GPU"):
         create_optimized_kirchhoff_migrator(velocity=VelocityModel(2000.0), config=MigrationConfig(),
                                             prefer_gpu=False, optimization_level="medium")
+
+
+def test_vti_leg_range_bound_keeps_the_anelliptic_product_in_fp32():
+    # The VTI kernels evaluate each leg as t = N rsq(N D), N = p^2 + 2 (1 + eta)
+    # tqs p + tqs^2, D = tqs + (1 + 2 eta) p (samples: tqs = (tau / 2 dt)^2,
+    # p = (h / (v dt))^2).  N D grows with p, tqs and eta, so its largest value
+    # of a call is at the latest output time, a leg at the aperture, the
+    # smallest velocity and the largest eta.  Every call the bound accepts
+    # keeps that product below the FP32 maximum; calls near the limit that it
+    # refuses would exceed it by a large factor once the leg times pass it.
+    from seisproc_amd.abi_types import PstmParams
+    from seisproc_amd.processors.migration.hip_ops import VTI_MAX_LEG2, vti_leg2_bound
+    rng = np.random.default_rng(7)
+    f32_max = float(np.finfo(np.float32).max)
+    accepted = refused = 0
+    for _ in range(20000):
+        dt = 10 ** rng.uniform(-4.5, -1.5)
+        n_tau = int(rng.integers(1, 4000))
+        dtau = dt * 10 ** rng.uniform(-0.5, 2.5)
+        tau0 = rng.uniform(0, 2.0)
+        ap = 10 ** rng.uniform(1, 5)
+        vmin = 10 ** rng.uniform(-1, 4)
+        eta = rng.uniform(-0.49, 2.0)
+        p = PstmParams.from_dict(dict(dt=dt, dtau=dtau, tau0=tau0, n_tau=n_tau, aperture_m=ap))
+        tqs = ((tau0 + (n_tau - 1) * dtau) / (2 * dt)) ** 2
+        pp = (ap / (vmin * dt)) ** 2
+        nd = (pp * pp + 2 * (1 + eta) * tqs * pp + tqs * tqs) * (tqs + (1 + 2 * eta) * pp)
+        if vti_leg2_bound(p, vmin, eta) < VTI_MAX_LEG2:
+            accepted += 1
+            assert nd < f32_max / 4, (dt, n_tau, dtau, tau0, ap, vmin, eta, nd)
+        else:
+            refused += 1
+    assert accepted > 1000 and refused > 1000
+    # the limit sits at 2^21-sample legs without anisotropy
+    p = PstmParams.from_dict(dict(dt=1e-3, dtau=1e-3, tau0=0.0, n_tau=1, aperture_m=2 ** 21 * 1e-3 * 1.0))
+    assert abs(vti_leg2_bound(p, 1.0, 0.0) - 2.0 ** 42) < 1e-6 * 2.0 ** 42
