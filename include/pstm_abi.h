@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define PSTM_ABI_VERSION 2
+#define PSTM_ABI_VERSION 3
 
 typedef enum {
   PSTM_OK = 0,
@@ -159,6 +159,27 @@ int pstm_plan_bands(const pstm_params_t* p);
 int pstm_forward_bands(const pstm_params_t* p, const void* ws, int32_t band_begin, int32_t band_end,
                        const void* traces, const float* trace_scale, const float* vel, const float* eta,
                        float* image, float* image_comp, float* fold, void* stream);
+
+/* Causal and anticausal integrals of every trace, the inputs of
+ * pstm_forward_integrated: int_c[t][s] = scale[t] * sum_{j <= s} trace[t][j]
+ * (FP32) and int_a[t][s] = sum_{k >= s} C[t][k] (FP64), [n_traces][n_samples]
+ * each (12 bytes per sample).  trace_scale may be NULL (all 1).  The triangle
+ * antialias filter works on these integrals ("triangle antialias via
+ * causal/anticausal integrated traces"); the reference's antialias module
+ * (processors/migration/antialias.py) computes filter weights only and is not
+ * wired into its migration path, so this has no counterpart there. */
+int pstm_integrate_traces(const pstm_params_t* p, const void* traces, const float* trace_scale,
+                          double* int_a, float* int_c, void* stream);
+
+/* pstm_forward_bands with the antialias filter (PSTM_FLAG_AA required) on
+ * traces integrated once by pstm_integrate_traces, instead of integrating each
+ * staged window in LDS for every image tile: same operator (the triangle
+ * taps are second differences, unchanged by the linear terms a later
+ * integration start adds), less work per tile.  Same bands, accumulation and
+ * outputs as pstm_forward_bands (kirchhoff_kernel.py:459-854). */
+int pstm_forward_integrated(const pstm_params_t* p, const void* ws, int32_t band_begin, int32_t band_end,
+                            const double* int_a, const float* int_c, const float* vel, const float* eta,
+                            float* image, float* image_comp, float* fold, void* stream);
 
 /* Adjoint of pstm_forward (Kirchhoff demigration / modelling):
  * traces_out += L^T image, FP32 [n_traces][n_samples].  Same masks, weights,

@@ -7,7 +7,7 @@ from __future__ import annotations
 import ctypes
 from typing import Any, Dict
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # pstm_status_t
 PSTM_OK = 0

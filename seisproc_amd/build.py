@@ -18,7 +18,7 @@ CSRC = HERE / "csrc"
 OUT_DIR = HERE / "_obj"
 LIB = HERE / "libpstm_hip.so"
 SOURCES = ["pstm_abi.hip", "pstm_plan.hip", "pstm_fwd_f32.hip", "pstm_fwd_bf16.hip", "pstm_fwd_f16.hip",
-           "pstm_adj.hip", "pstm_grad.hip", "pstm_static.hip"]
+           "pstm_adj.hip", "pstm_grad.hip", "pstm_static.hip", "pstm_integ.hip"]
 ARCH = os.environ.get("PSTM_OFFLOAD_ARCH", "gfx950")
 # Standalone probe (not part of the library): VALU issue rates of the
 # instruction kinds the tile kernels use, ./seisproc_amd/valu_rates

@@ -273,9 +273,12 @@ int pstm_forward(const pstm_params_t* p, const void* ws, const void* traces,
   return pstm_forward_bands(p, ws, 0, nb, traces, trace_scale, vel, eta, image, image_comp, fold, stream);
 }
 
-int pstm_forward_bands(const pstm_params_t* p, const void* ws, int32_t band_begin, int32_t band_end,
-                       const void* traces, const float* trace_scale, const float* vel, const float* eta,
-                       float* image, float* image_comp, float* fold, void* stream) {
+// Forward over offset bands [band_begin, band_end): from the traces, or with
+// int_a/int_c from their integrals (antialias).
+static int forward_common(const pstm_params_t* p, const void* ws, int32_t band_begin, int32_t band_end,
+                          const void* traces, const float* trace_scale, const double* int_a, const float* int_c,
+                          const float* vel, const float* eta, float* image, float* image_comp, float* fold,
+                          void* stream) {
   PlanLayout L;
   TileArgs a;
   int rc = prep(p, ws, &L, &a);
@@ -284,11 +287,13 @@ int pstm_forward_bands(const pstm_params_t* p, const void* ws, int32_t band_begi
     return set_error(PSTM_ERR_INVALID, "offset bands [%d, %d) outside [0, %d)", band_begin, band_end, L.n_bands);
   a.band0 = band_begin;
   a.band1 = band_end;
-  if (!vel || !image || (p->n_traces > 0 && !traces))
+  if (!vel || !image || (p->n_traces > 0 && !traces && !int_a))
     return set_error(PSTM_ERR_INVALID, "NULL traces, vel or image");
   if ((p->flags & PSTM_FLAG_VTI) && !eta) return set_error(PSTM_ERR_INVALID, "PSTM_FLAG_VTI needs eta");
   a.traces = traces;
   a.scale = trace_scale;
+  a.int_a = int_a;
+  a.int_c = int_c;
   a.vel = vel;
   a.eta = eta;
   a.image = image;
@@ -296,6 +301,33 @@ int pstm_forward_bands(const pstm_params_t* p, const void* ws, int32_t band_begi
   a.fold = fold;
   if (p->n_traces == 0) return PSTM_OK;
   return launch_forward(p, L, a, (hipStream_t)stream);
+}
+
+int pstm_forward_bands(const pstm_params_t* p, const void* ws, int32_t band_begin, int32_t band_end,
+                       const void* traces, const float* trace_scale, const float* vel, const float* eta,
+                       float* image, float* image_comp, float* fold, void* stream) {
+  return forward_common(p, ws, band_begin, band_end, traces, trace_scale, nullptr, nullptr, vel, eta, image,
+                        image_comp, fold, stream);
+}
+
+int pstm_integrate_traces(const pstm_params_t* p, const void* traces, const float* trace_scale, double* int_a,
+                          float* int_c, void* stream) {
+  int rc = validate(p);
+  if (rc != PSTM_OK) return rc;
+  if (p->n_traces > 0 && (!traces || !int_a || !int_c))
+    return set_error(PSTM_ERR_INVALID, "NULL traces or integral buffers");
+  if (p->n_traces == 0) return PSTM_OK;
+  return launch_integrate(p, traces, trace_scale, int_a, int_c, (hipStream_t)stream);
+}
+
+int pstm_forward_integrated(const pstm_params_t* p, const void* ws, int32_t band_begin, int32_t band_end,
+                            const double* int_a, const float* int_c, const float* vel, const float* eta,
+                            float* image, float* image_comp, float* fold, void* stream) {
+  if (p && !(p->flags & PSTM_FLAG_AA))
+    return set_error(PSTM_ERR_INVALID, "pstm_forward_integrated is the antialias forward (PSTM_FLAG_AA)");
+  if (p && p->n_traces > 0 && (!int_a || !int_c)) return set_error(PSTM_ERR_INVALID, "NULL integral buffers");
+  return forward_common(p, ws, band_begin, band_end, nullptr, nullptr, int_a, int_c, vel, eta, image, image_comp,
+                        fold, stream);
 }
 
 int pstm_adjoint(const pstm_params_t* p, const void* ws, const float* image, const float* vel,

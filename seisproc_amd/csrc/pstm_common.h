@@ -208,6 +208,12 @@ struct TileArgs {
   const float* image_in;     // adjoint input image or gradient upstream
   float* traces_out;         // adjoint output
   float* vel_grad;           // gradient output
+  // antialias forward with traces integrated once per call (or NULL: each
+  // staged window is integrated in LDS): [trace][sample] FP64 double integral
+  // A(s) = sum_{k >= s} C(k) and FP32 single integral C(s) = sum_{j <= s} x(j),
+  // storage scale applied
+  const double* int_a;
+  const float* int_c;
 };
 
 }  // namespace pstm

@@ -24,6 +24,9 @@ int quantize(const float* x, int64_t n_traces, int32_t n_samples, int32_t dtype,
 void fill_tile_args(const pstm_params_t* p, const PlanLayout& L, const void* ws, TileArgs* a);
 
 int launch_forward(const pstm_params_t* p, const PlanLayout& L, const TileArgs& a, hipStream_t s);
+// Causal and anticausal integrals of every trace (antialias forward inputs).
+int launch_integrate(const pstm_params_t* p, const void* traces, const float* scale, double* int_a,
+                     float* int_c, hipStream_t s);
 int launch_adjoint(const pstm_params_t* p, const PlanLayout& L, const TileArgs& a, hipStream_t s);
 int launch_vel_grad(const pstm_params_t* p, const PlanLayout& L, const TileArgs& a,
                     hipStream_t s);

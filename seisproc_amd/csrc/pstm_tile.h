@@ -653,6 +653,37 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
               }
               continue;
             }
+            if constexpr (AA && MODE == MODE_FWD) {
+              if (a.int_a) {
+                // traces integrated once per call: copy the window of A and C
+                // (storage scale already applied; the offset taper scales both),
+                // constant before the record, and past its end C = C(n-1) = A(n-1)
+                // with A falling linearly: A(n-1+k) = (1 - k) A(n-1)
+                const float ot = use_taper ? offset_taper(a.offs[S.q_j[e]], a) : 1.f;
+                const double* const ra = a.int_a + row;
+                const float* const rc = a.int_c + row;
+                const int n = a.n_samples;
+                for (int s = lane; s < len; s += 64) {
+                  const int src = w0 + s;
+                  double av;
+                  float cv;
+                  if (src < 0) {
+                    av = ra[0];
+                    cv = 0.f;
+                  } else if (src < n) {
+                    av = ra[src];
+                    cv = rc[src];
+                  } else {
+                    const double ce = ra[n - 1];
+                    av = (double)(n - src) * ce;
+                    cv = (float)ce;
+                  }
+                  Wa[SA * (off + s)] = av * (double)ot;
+                  Wc[SC * (off + s)] = cv * ot;
+                }
+                continue;
+              }
+            }
             // in-record part [lo, hi) of the window; antialias windows also
             // hold zero padding outside the record
             const int lo = max(w0, 0), hi = min(w0 + len, a.n_samples);
@@ -717,10 +748,13 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
         }
         __syncthreads();
         if constexpr (AA && MODE != MODE_ADJ) {
-          // double integration (causal, then anticausal) of each window
-          for (int b = wave; b < nb; b += kWaves)
-            integrate_window<SA, SC>(Wa + SA * S.b_off[b], Wc + SC * S.b_off[b], S.q_len[qs + b], lane);
-          __syncthreads();
+          // double integration (causal, then anticausal) of each window,
+          // unless the traces come integrated
+          if (!a.int_a) {
+            for (int b = wave; b < nb; b += kWaves)
+              integrate_window<SA, SC>(Wa + SA * S.b_off[b], Wc + SC * S.b_off[b], S.q_len[qs + b], lane);
+            __syncthreads();
+          }
         }
 
         // -- contributions of the traces that reach this wave's samples

@@ -32,6 +32,9 @@ SYMBOLS = {
     "pstm_plan_bands": (ctypes.c_int, [_P]),
     "pstm_forward_bands": (ctypes.c_int, [_P, _V, ctypes.c_int32, ctypes.c_int32, _V, _V, _V, _V, _V, _V, _V,
                                           _V]),
+    "pstm_integrate_traces": (ctypes.c_int, [_P, _V, _V, _V, _V, _V]),
+    "pstm_forward_integrated": (ctypes.c_int, [_P, _V, ctypes.c_int32, ctypes.c_int32, _V, _V, _V, _V, _V, _V,
+                                               _V, _V]),
     "pstm_adjoint": (ctypes.c_int, [_P, _V, _V, _V, _V, _V, _V]),
     "pstm_vel_grad": (ctypes.c_int, [_P, _V, _V, _V, _V, _V, _V, _V]),
     "pstm_migrate_static": (ctypes.c_int, [_P, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V]),

@@ -158,7 +158,8 @@ class PstmPlan:
                 eta: Optional[torch.Tensor] = None, image: Optional[torch.Tensor] = None,
                 comp: Optional[torch.Tensor] = None, fold: Optional[torch.Tensor] = None,
                 with_fold: bool = True, flags: Optional[int] = None,
-                progress_callback: Optional[Callable[[float, str], None]] = None):
+                progress_callback: Optional[Callable[[float, str], None]] = None,
+                integrated: Optional[bool] = None):
         """Accumulate the migrated image (and fold) of ``traces``.
 
         traces: (n_traces, n_samples) float32 / bfloat16 / float16 (trace-major)
@@ -171,6 +172,11 @@ class PstmPlan:
             band and the stream is synchronised between them; the sums are
             the same as one call's), like the reference kernel's callback
             (processors/migration/kirchhoff_kernel.py:760-774).
+        integrated: with the antialias filter, integrate every trace once per
+            call (pstm_integrate_traces, 12 bytes per sample of device memory
+            for the call) and migrate from the integrals
+            (pstm_forward_integrated) -- the default -- or, with False,
+            integrate each staged window per image tile (pstm_forward).
         Returns (image, fold) -- fold is None when with_fold is False.
         """
         p = self.params
@@ -194,16 +200,28 @@ class PstmPlan:
             fold = torch.zeros(self.image_shape, dtype=torch.float32, device=self.device)
         if fold is not None:
             _need_cuda(fold, "fold", torch.float32, self.image_shape)
-        args = (_ptr(traces), _ptr(scale), _ptr(vel), _ptr(eta if cp.flags & FLAG_VTI else None), _ptr(image),
-                _ptr(comp), _ptr(fold), ctypes.c_void_p(_stream_handle()))
-        if progress_callback is None:
-            check(self.lib.pstm_forward(ctypes.byref(cp), _ptr(self.workspace), *args), "pstm_forward", self.lib)
-            return image, fold
+        stream = ctypes.c_void_p(_stream_handle())
+        outs = (_ptr(vel), _ptr(eta if cp.flags & FLAG_VTI else None), _ptr(image), _ptr(comp), _ptr(fold), stream)
+        if (cp.flags & FLAG_AA) and integrated is not False and p.n_traces > 0:
+            int_a = torch.empty((p.n_traces, p.n_samples), dtype=torch.float64, device=self.device)
+            int_c = torch.empty((p.n_traces, p.n_samples), dtype=torch.float32, device=self.device)
+            check(self.lib.pstm_integrate_traces(ctypes.byref(cp), _ptr(traces), _ptr(scale), _ptr(int_a),
+                                                 _ptr(int_c), stream), "pstm_integrate_traces", self.lib)
+
+            def run(b0, b1):
+                check(self.lib.pstm_forward_integrated(ctypes.byref(cp), _ptr(self.workspace), b0, b1, _ptr(int_a),
+                                                       _ptr(int_c), *outs), "pstm_forward_integrated", self.lib)
+        else:
+            def run(b0, b1):
+                check(self.lib.pstm_forward_bands(ctypes.byref(cp), _ptr(self.workspace), b0, b1, _ptr(traces),
+                                                  _ptr(scale), *outs), "pstm_forward_bands", self.lib)
         n_bands = self.lib.pstm_plan_bands(ctypes.byref(cp))
         check(min(n_bands, 0), "pstm_plan_bands", self.lib)
+        if progress_callback is None:
+            run(0, n_bands)
+            return image, fold
         for band in range(n_bands):
-            check(self.lib.pstm_forward_bands(ctypes.byref(cp), _ptr(self.workspace), band, band + 1, *args),
-                  "pstm_forward_bands", self.lib)
+            run(band, band + 1)
             torch.cuda.current_stream(self.device).synchronize()
             progress_callback((band + 1) / n_bands, f"offset band {band + 1}/{n_bands} migrated")
         return image, fold

@@ -268,3 +268,70 @@ def test_forward_with_segy_velocity_cube(tmp_path):
     sx, sy, rx, ry = s.numpy_geometry()
     ref, rfold = oracle.forward(pp, s.traces.double().numpy(), sx, sy, rx, ry, vel.double().cpu().numpy())
     _compare("segy-velocity-cube", img, fold, ref, rfold)
+
+
+AA_PATH_CASES = [
+    ("aa", dict(flags=FLAG_AA), dict(), "fp32"),
+    ("aa-vti-lateral", dict(flags=FLAG_AA | FLAG_VTI, lateral=True), dict(), "fp32"),
+    ("aa-taper-offset-window", dict(flags=FLAG_AA, lateral=True),
+     dict(flags=FLAG_AA | FLAG_TAPER, min_offset_m=100.0, max_offset_m=500.0, taper_width=0.3), "fp32"),
+    ("aa-bf16-scaled", dict(flags=FLAG_AA), dict(), "bf16"),
+    ("aa-fp16-kahan", dict(flags=FLAG_AA), dict(flags=FLAG_AA | FLAG_KAHAN), "fp16"),
+]
+
+
+@pytest.mark.parametrize("name,survey_kw,param_kw,form", AA_PATH_CASES, ids=[c[0] for c in AA_PATH_CASES])
+def test_antialias_from_integrated_traces_equals_per_window_integration(name, survey_kw, param_kw, form):
+    # The antialias forward from traces integrated once per call
+    # (pstm_integrate_traces + pstm_forward_integrated, the default) and from
+    # windows integrated per tile in LDS (pstm_forward) apply the same
+    # operator: both match the oracle (on the dequantised traces for the
+    # low-precision forms) and each other to FP32 rounding.  The records end
+    # before the deepest far-offset times, so taps also read the linear
+    # extension past the record end.
+    from seisproc_amd.processors.migration.hip_ops import quantize_traces
+    s = S.tiny_survey(seed=21, n_bins=14, n_classes=3, n_samples=251, n_img=12, aperture=400.0, **survey_kw)
+    p = s.params.replace(vel_mode=1 if s.vel.dim() == 3 else 0, **param_kw)
+    plan = _plan(s, **param_kw)
+    tr = s.traces.to(DEV)
+    sc = None
+    ref_traces = None
+    if form != "fp32":
+        tr, sc = quantize_traces(tr, torch.bfloat16 if form == "bf16" else torch.float16)
+        ref_traces = (tr.float() * sc[:, None]).cpu()
+    eta = s.eta.to(DEV) if s.eta is not None else None
+    out = {}
+    for integrated in (True, False):
+        comp = torch.zeros(plan.image_shape, device=DEV) if p.flags & FLAG_KAHAN else None
+        img, fold = plan.forward(tr, s.vel.to(DEV), scale=sc, eta=eta, comp=comp, integrated=integrated)
+        torch.cuda.synchronize()
+        out[integrated] = (img + comp if comp is not None else img, fold)
+    ref, rfold = _oracle(s, p, traces=ref_traces)
+    for integrated, (img, fold) in out.items():
+        _compare(f"{name}-{'integrated' if integrated else 'per-window'}", img, fold, ref, rfold)
+    a, b = (out[k][0].double().cpu().numpy() for k in (True, False))
+    assert torch.equal(out[True][1], out[False][1])  # same masks: same fold
+    d = np.linalg.norm(a - b) / np.linalg.norm(b)
+    print(f"[{name}] integrated vs per-window rel diff {d:.3e}")
+    assert d <= 1e-5
+
+
+@pytest.mark.parametrize("n_samples,dtau,flags", [(8001, 0.016, FLAG_AA), (8001, 0.016, FLAG_AA | FLAG_VTI)])
+def test_antialias_integrated_traces_long_segmented_windows(n_samples, dtau, flags):
+    # records whose windows exceed every form's LDS share are staged in
+    # segments; from integrated traces each segment is a copy
+    from seisproc_amd.processors.migration.hip_ops import PstmPlan
+    s = S.tiny_survey(seed=19, n_bins=6, n_classes=2, n_samples=n_samples, n_img=4, aperture=2500.0,
+                      flags=flags & (FLAG_AA | FLAG_VTI))
+    n_tau = int(round((n_samples - 1) * 0.0005 / dtau))
+    p = s.params.replace(dt=0.0005, n_tau=n_tau, dtau=dtau, flags=flags, max_angle_deg=85.0,
+                         vel_mode=1 if s.vel.dim() == 3 else 0)
+    vel = torch.full((n_tau,), 1500.0) if s.vel.dim() == 1 else torch.full((4, 4, n_tau), 1500.0)
+    eta = None if s.eta is None else s.eta[:, :, :n_tau].contiguous()
+    plan = PstmPlan(p, s.src_xy, s.rcv_xy)
+    img, fold = plan.forward(s.traces.to(DEV), vel.to(DEV), eta=None if eta is None else eta.to(DEV),
+                             integrated=True)
+    sx, sy, rx, ry = s.numpy_geometry()
+    ref, rfold = oracle.forward(p, s.traces.double().numpy(), sx, sy, rx, ry, vel.double().numpy(),
+                                eta=None if eta is None else eta.double().numpy())
+    _compare(f"integrated-segmented-{n_samples}-flags{flags}", img, fold, ref, rfold)
