@@ -663,23 +663,29 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
                 const double* const ra = a.int_a + row;
                 const float* const rc = a.int_c + row;
                 const int n = a.n_samples;
-                for (int s = lane; s < len; s += 64) {
-                  const int src = w0 + s;
-                  double av;
-                  float cv;
-                  if (src < 0) {
-                    av = ra[0];
-                    cv = 0.f;
-                  } else if (src < n) {
-                    av = ra[src];
-                    cv = rc[src];
-                  } else {
-                    const double ce = ra[n - 1];
-                    av = (double)(n - src) * ce;
-                    cv = (float)ce;
+                // four independent pairs of loads per lane in flight (indexes
+                // clamped into the record), then the extension and the stores
+                for (int s0 = 0; s0 < len; s0 += 4 * 64) {
+                  double av[4];
+                  float cv[4];
+#pragma unroll
+                  for (int u = 0; u < 4; ++u) {
+                    const int src = min(max(w0 + s0 + lane + 64 * u, 0), n - 1);
+                    av[u] = ra[src];
+                    cv[u] = rc[src];
                   }
-                  Wa[SA * (off + s)] = av * (double)ot;
-                  Wc[SC * (off + s)] = cv * ot;
+#pragma unroll
+                  for (int u = 0; u < 4; ++u) {
+                    const int s = s0 + lane + 64 * u;
+                    const int src = w0 + s;
+                    if (s < len) {
+                      // past the end: A(n-1+k) = (1 - k) A(n-1), C = A(n-1)
+                      const double A = src < n ? av[u] : (double)(n - src) * av[u];
+                      const float C = src < 0 ? 0.f : (src < n ? cv[u] : (float)av[u]);
+                      Wa[SA * (off + s)] = A * (double)ot;
+                      Wc[SC * (off + s)] = C * ot;
+                    }
+                  }
                 }
                 continue;
               }
