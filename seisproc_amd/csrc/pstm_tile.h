@@ -663,6 +663,28 @@ __global__ __launch_bounds__(kThreads, OCC) void k_tile(TileArgs a) {
                 const double* const ra = a.int_a + row;
                 const float* const rc = a.int_c + row;
                 const int n = a.n_samples;
+                if (w0 >= 0 && w0 + len <= n && !use_taper) {
+                  // the window lies inside the record: a plain copy
+                  for (int s0 = 0; s0 < len; s0 += 4 * 64) {
+                    double av[4];
+                    float cv[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                      const int src = w0 + min(s0 + lane + 64 * u, len - 1);
+                      av[u] = ra[src];
+                      cv[u] = rc[src];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                      const int s = s0 + lane + 64 * u;
+                      if (s < len) {
+                        Wa[SA * (off + s)] = av[u];
+                        Wc[SC * (off + s)] = cv[u];
+                      }
+                    }
+                  }
+                  continue;
+                }
                 // four independent pairs of loads per lane in flight (indexes
                 // clamped into the record), then the extension and the stores
                 for (int s0 = 0; s0 < len; s0 += 4 * 64) {
