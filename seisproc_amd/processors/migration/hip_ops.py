@@ -175,8 +175,9 @@ class PstmPlan:
         integrated: with the antialias filter, integrate every trace once per
             call (pstm_integrate_traces, 12 bytes per sample of device memory
             for the call) and migrate from the integrals
-            (pstm_forward_integrated) -- the default -- or, with False,
-            integrate each staged window per image tile (pstm_forward).
+            (pstm_forward_integrated), or, with False, integrate each staged
+            window per image tile (pstm_forward).  The default (None) takes the
+            first unless the device cannot hold the integrals.
         Returns (image, fold) -- fold is None when with_fold is False.
         """
         p = self.params
@@ -202,9 +203,16 @@ class PstmPlan:
             _need_cuda(fold, "fold", torch.float32, self.image_shape)
         stream = ctypes.c_void_p(_stream_handle())
         outs = (_ptr(vel), _ptr(eta if cp.flags & FLAG_VTI else None), _ptr(image), _ptr(comp), _ptr(fold), stream)
+        int_a = int_c = None
         if (cp.flags & FLAG_AA) and integrated is not False and p.n_traces > 0:
-            int_a = torch.empty((p.n_traces, p.n_samples), dtype=torch.float64, device=self.device)
-            int_c = torch.empty((p.n_traces, p.n_samples), dtype=torch.float32, device=self.device)
+            try:
+                int_a = torch.empty((p.n_traces, p.n_samples), dtype=torch.float64, device=self.device)
+                int_c = torch.empty((p.n_traces, p.n_samples), dtype=torch.float32, device=self.device)
+            except torch.cuda.OutOfMemoryError:
+                if integrated:  # asked for explicitly
+                    raise
+                int_a = int_c = None  # no room for the integrals: integrate per window instead
+        if int_a is not None:
             check(self.lib.pstm_integrate_traces(ctypes.byref(cp), _ptr(traces), _ptr(scale), _ptr(int_a),
                                                  _ptr(int_c), stream), "pstm_integrate_traces", self.lib)
 
