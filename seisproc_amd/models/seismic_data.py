@@ -47,19 +47,11 @@ class SeismicData:
     def get_trace_axis(self) -> np.ndarray:
         return np.arange(self.n_traces)
 
-    @property
-    def coordinate_units(self) -> str:
-        return self.metadata.get("coordinate_units", "meters")
-
-    @property
-    def unit_symbol(self) -> str:
-        return "m" if self.coordinate_units == "meters" else "ft"
-
     def copy(self) -> "SeismicData":
         return SeismicData(traces=self.traces.copy(), sample_rate=self.sample_rate,
                            headers={k: v.copy() for k, v in self.headers.items()} if self.headers else None,
                            metadata=dict(self.metadata))
 
     def __repr__(self) -> str:
-        return (f"SeismicData(n_samples={self.n_samples}, n_traces={self.n_traces}, "
-                f"sample_rate={self.sample_rate}ms, nyquist={self.nyquist_freq:.1f}Hz)")
+        # the dataclass default would print every sample
+        return f"<SeismicData {self.n_traces} traces x {self.n_samples} samples at {self.sample_rate:g} ms>"

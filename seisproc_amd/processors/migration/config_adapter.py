@@ -130,17 +130,22 @@ class ConfigAdapter:
         return True, "Configuration valid"
 
     def get_summary(self) -> str:
+        """A log-ready description of the extracted parameters (the monitor
+        dialog logs it, reference views/migration_monitor_dialog.py:140): one
+        ``name: value`` line each for the grid, origin, bins, azimuth, time
+        axis, velocity and aperture."""
         p = self.params
-        return "\n".join([
-            "Migration Parameters:",
-            f"  Output grid: {p.n_il} IL x {p.n_xl} XL",
-            f"  Origin: ({p.origin_x:.1f}, {p.origin_y:.1f})",
-            f"  Bin sizes: {p.il_spacing:.1f}m IL, {p.xl_spacing:.1f}m XL",
-            f"  Azimuth: {p.azimuth_deg:.1f}°",
-            f"  Time: {p.t_min_ms:.0f} - {p.t_min_ms + (p.n_samples - 1) * p.dt_ms:.0f} ms @ {p.dt_ms:.1f}ms",
-            f"  Velocity: {p.velocity_mps:.0f} m/s",
-            f"  Aperture: {p.max_aperture_m:.0f}m, angle: {p.max_angle_deg:.0f}°",
-        ])
+        t_end = p.t_min_ms + (p.n_samples - 1) * p.dt_ms
+        rows = (
+            ("grid", f"{p.n_il} x {p.n_xl} (inline x crossline), {p.n_samples} samples"),
+            ("origin", f"x {p.origin_x:.1f} m, y {p.origin_y:.1f} m"),
+            ("bins", f"{p.il_spacing:g} m inline, {p.xl_spacing:g} m crossline"),
+            ("azimuth", f"{p.azimuth_deg:g} deg"),
+            ("time", f"{p.t_min_ms:g}..{t_end:g} ms every {p.dt_ms:g} ms"),
+            ("velocity", f"{p.velocity_mps:g} m/s"),
+            ("aperture", f"{p.max_aperture_m:g} m, angle {p.max_angle_deg:g} deg"),
+        )
+        return "PSTM parameters\n" + "\n".join(f"  {k:<9} {v}" for k, v in rows)
 
 
 def create_adapter_from_wizard(config: Dict[str, Any]) -> ConfigAdapter:

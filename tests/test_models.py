@@ -65,6 +65,18 @@ def test_adapter_engine_params_and_validation():
     assert not ok and "large" in msg.lower()
 
 
+
+def test_adapter_summary_names_each_parameter():
+    # the reference's summary checks (tests/test_config_adapter.py:366-410): the
+    # grid size, velocity, aperture and bin size appear, and each section is named
+    s = ConfigAdapter(WIZARD).get_summary()
+    assert isinstance(s, str)
+    for value in ("100", "3000", "5000", "25"):
+        assert value in s
+    s = ConfigAdapter({"velocity_v0": 2500.0}).get_summary().lower()
+    for word in ("grid", "origin", "velocity", "aperture"):
+        assert word in s
+
 # ---------------------------------------------------------------- MigrationConfig
 def test_migration_config_validation_and_roundtrip():
     with pytest.raises(ValueError, match="max_angle_deg"):
@@ -207,6 +219,7 @@ def test_seismic_data_contract():
     assert d.traces.dtype == np.float32 and d.n_samples == 11 and d.duration == 20.0 and d.nyquist_freq == 250.0
     with pytest.raises(ValueError):
         SeismicData(traces=np.zeros(5), sample_rate=2.0)
+    assert "3 traces" in repr(d) and "11 samples" in repr(d)
 
 
 def test_pstm_processor_params_roundtrip_without_gpu():
