@@ -7,13 +7,15 @@ information from the file itself, following the SEG-Y revision 1/2 layout:
   bytes 1-3200      textual header (skipped; EBCDIC or ASCII)
   bytes 3201-3600   binary header, big-endian unless the revision-2
                     byte-order word (bytes 3297-3300) reads 0x04030201
-                      3217-3218  sample interval (microseconds)
-                      3221-3222  samples per trace
+                      3217-3218  sample interval (microseconds, unsigned)
+                      3221-3222  samples per trace (unsigned)
                       3225-3226  sample format code
                       3505-3506  number of extended textual headers
   then, per trace, a 240-byte trace header and the samples
                       21-24    CDP ensemble number
                       109-110  delay recording time (ms)
+                      115-116  samples in this trace, 117-118 its interval (us);
+                               used when the binary header holds zero
                       181-184  CDP X, 185-188 CDP Y
                       189-192  inline (3D), 193-196 crossline (3D)
 
@@ -99,14 +101,20 @@ def read_segy(path: Union[str, Path], inline_byte: Optional[int] = None,
     little = order == 0x04030201
     e = "<" if little else ">"
     i2 = lambda off: int(binh[off:off + 2].view(e + "i2")[0])
-    interval_us, n_samples, fmt = i2(16), i2(20), i2(24)
+    u2 = lambda buf, off: int(buf[off:off + 2].view(e + "u2")[0])
+    interval_us, n_samples, fmt = u2(binh, 16), u2(binh, 20), i2(24)
     n_ext = i2(304)
     if fmt not in _FORMATS:
         raise ValueError(f"{path}: unsupported SEG-Y sample format code {fmt}")
-    if n_samples <= 0:
-        raise ValueError(f"{path}: binary header gives {n_samples} samples per trace")
     kind, nbytes = _FORMATS[fmt]
     start = TEXT_BYTES + BIN_BYTES + max(n_ext, 0) * TEXT_BYTES
+    if (n_samples == 0 or interval_us == 0) and raw.size >= start + TRACE_HEADER_BYTES:
+        # zero in the binary header: take the first trace header's values
+        th0 = raw[start:start + TRACE_HEADER_BYTES]
+        n_samples = n_samples or u2(th0, 114)
+        interval_us = interval_us or u2(th0, 116)
+    if n_samples <= 0:
+        raise ValueError(f"{path}: neither the binary header nor the first trace header gives a sample count")
     rec = TRACE_HEADER_BYTES + n_samples * nbytes
     body = raw[start:]
     n_tr = body.size // rec
@@ -151,8 +159,8 @@ def write_segy(path: Union[str, Path], samples: np.ndarray, interval_s: float, *
     def put(buf, off, val, t):
         buf[off:off + np.dtype(t).itemsize] = np.frombuffer(np.asarray(val, dtype=e + t).tobytes(), dtype=np.uint8)
 
-    put(binh, 16, int(round(interval_s * 1e6)), "i2")
-    put(binh, 20, n_s, "i2")
+    put(binh, 16, int(round(interval_s * 1e6)), "u2")
+    put(binh, 20, n_s, "u2")
     put(binh, 24, format_code, "i2")
     put(binh, 96, 0x01020304, "u4")
     put(binh, 300, 0x0200, "u2")  # revision 2.0

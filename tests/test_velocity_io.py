@@ -154,6 +154,31 @@ def test_read_velocity_segy_cdp_line_and_single_trace(tmp_path):
     np.testing.assert_allclose(m1.data, v[0], rtol=0, atol=1e-3)
 
 
+
+def test_segy_header_counts_unsigned_and_trace_header_fallback(tmp_path):
+    # samples per trace and the interval are unsigned 16-bit fields: a
+    # 40,000-sample record reads back whole; a binary header holding zero in
+    # either field defers to the first trace header (bytes 115-118)
+    from seisproc_amd.utils.segy import BIN_BYTES, TEXT_BYTES, read_segy, write_segy
+    v = 1500.0 + np.arange(40000, dtype=np.float64)[None, :] * 0.25 + np.array([[0.0], [7.0]])
+    f = tmp_path / "long.sgy"
+    write_segy(f, v, 0.0005, format_code=5)
+    vol = read_segy(f)
+    assert vol.samples.shape == (2, 40000) and vol.interval_s == 0.0005
+    np.testing.assert_array_equal(vol.samples, v.astype(np.float32))
+    raw = bytearray(f.read_bytes())
+    raw[TEXT_BYTES + 16:TEXT_BYTES + 18] = b"\0\0"  # interval
+    raw[TEXT_BYTES + 20:TEXT_BYTES + 22] = b"\0\0"  # samples per trace
+    g = tmp_path / "zero_bin.sgy"
+    g.write_bytes(bytes(raw))
+    vol0 = read_segy(g)
+    assert vol0.samples.shape == (2, 40000) and vol0.interval_s == 0.0005
+    np.testing.assert_array_equal(vol0.samples, vol.samples)
+    raw[TEXT_BYTES + BIN_BYTES + 114:TEXT_BYTES + BIN_BYTES + 116] = b"\0\0"  # and trace 0's count
+    g.write_bytes(bytes(raw))
+    with pytest.raises(ValueError, match="sample count"):
+        read_segy(g)
+
 def test_preview_and_info_to_model(tmp_path):
     from seisproc_amd.utils.segy import write_segy
     from seisproc_amd.utils.velocity_io import preview_velocity_file, velocity_info_to_model
