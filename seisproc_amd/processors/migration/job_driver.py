@@ -37,7 +37,7 @@ from typing import Callable, Dict, Iterator, List, Optional, Tuple
 import numpy as np
 import torch
 
-from ...abi_types import PstmParams
+from ...abi_types import FLAG_KAHAN, PstmParams
 from ...models.binning import BinningTable
 
 logger = logging.getLogger(__name__)
@@ -244,7 +244,7 @@ class MigrationJobRunner:
         h.update(json.dumps(self.params.to_dict(), sort_keys=True).encode())
         h.update(json.dumps(source.binning.to_dict(), sort_keys=True).encode())
         h.update(json.dumps(source.bin_summary(), sort_keys=True).encode())
-        h.update(self.trace_dtype.encode())
+        h.update(repr((self.trace_dtype, bool(self.stack))).encode())
         h.update(np.ascontiguousarray(self.velocity.detach().float().cpu().numpy()).tobytes())
         if self.eta is not None:
             h.update(np.ascontiguousarray(self.eta.detach().float().cpu().numpy()).tobytes())
@@ -291,7 +291,7 @@ class MigrationJobRunner:
         scale = None
         if self.trace_dtype in ("bf16", "fp16"):
             traces, scale = quantize_traces(traces, torch.bfloat16 if self.trace_dtype == "bf16" else torch.float16)
-        comp = torch.zeros(plan.image_shape, dtype=torch.float32, device=self.device) if p.flags & 4 else None
+        comp = torch.zeros(plan.image_shape, dtype=torch.float32, device=self.device) if p.flags & FLAG_KAHAN else None
         img, fold = plan.forward(traces, self.velocity, scale=scale, eta=self.eta, comp=comp)
         if comp is not None:
             img += comp

@@ -121,6 +121,13 @@ def test_checkpoint_of_another_job_is_refused(tmp_path):
     # an explicit restart ignores the old state
     res = other.run(ArrayBinSource(tr, *geo, t), resume=False)
     assert len(res.completed) == 2 and not res.skipped
+    # a checkpoint kept without a running stack cannot resume a job that keeps one
+    nostack = tmp_path / "nostack"
+    MigrationJobRunner(p, vel, nostack, device=torch.device("cpu"), migrate_fn=_fake_migrate, stack=False) \
+        .run(ArrayBinSource(tr, *geo, t))
+    with pytest.raises(RuntimeError, match="different job"):
+        MigrationJobRunner(p, vel, nostack, device=torch.device("cpu"), migrate_fn=_fake_migrate) \
+            .run(ArrayBinSource(tr, *geo, t))
 
 
 def test_runner_needs_the_gpu_without_an_injected_migration(tmp_path):
