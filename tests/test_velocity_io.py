@@ -203,6 +203,52 @@ def test_preview_and_info_to_model(tmp_path):
         velocity_info_to_model(bad)
 
 
+
+def test_preview_and_info_to_model_tables_json_npz(tmp_path):
+    # the preview of each remaining format and the model built from it (reference
+    # utils/velocity_io.py:1175-1290): CDP tables through their reader, inline-
+    # crossline tables as v(t, inline) on the first inline's times, JSON and .npz
+    # through theirs; a file of one numeric column is not a velocity table
+    from seisproc_amd.utils.velocity_io import (VelocityFileFormat, preview_velocity_file, velocity_info_to_model,
+                                                write_velocity_json, write_velocity_npy)
+    cdp = tmp_path / "cdp.txt"
+    cdp.write_text("".join(f"{c} {t} {1500 + c + t}\n" for c in (30, 10, 20) for t in (0, 500, 1000)))
+    i1 = preview_velocity_file(cdp)
+    assert i1.is_valid and i1.format is VelocityFileFormat.ASCII_CDPTV
+    assert (i1.n_locations, i1.n_time_samples, i1.cdp_range) == (3, 3, (10, 30)) and i1.sample_interval == 500.0
+    m1 = velocity_info_to_model(i1)
+    np.testing.assert_allclose(m1.z_axis, [0.0, 0.5, 1.0])
+    np.testing.assert_allclose(m1.data[:, 0], [1510, 2010, 2510])  # sorted by CDP
+
+    ilxl = tmp_path / "ilxl.txt"
+    rows = [(il, xl, t, 1500 + 10 * il + xl + t) for il in (5, 6) for xl in (1, 2) for t in (0, 1000, 2000)]
+    ilxl.write_text("".join("%d %d %d %d\n" % r for r in rows))
+    i2 = preview_velocity_file(ilxl)
+    assert i2.is_valid and i2.format is VelocityFileFormat.ASCII_ILXLTV
+    assert (i2.n_locations, i2.n_time_samples, i2.inline_range, i2.xline_range) == (4, 3, (5, 6), (1, 2))
+    m2 = velocity_info_to_model(i2, velocity_unit="km/s")
+    # the first inline's times, every crossline's included (the reference's axis)
+    np.testing.assert_allclose(m2.z_axis, [0.0, 0.0, 1.0, 1.0, 2.0, 2.0])
+    np.testing.assert_array_equal(m2.x_axis, [5, 6])
+    # (np.interp over the repeated times picks one crossline's value)
+    assert m2.data.shape == (6, 2) and 1000.0 * 1561 <= m2.data[0, 1] <= 1000.0 * 1562
+
+    vm = VelocityModel(data=np.array([1500.0, 2000.0, 2600.0], np.float32), z_axis=np.array([0.0, 1.0, 2.0]))
+    js, nz = tmp_path / "v.json", tmp_path / "v.npz"
+    write_velocity_json(vm, js)
+    write_velocity_npy(vm, nz)
+    for f, fmt in ((js, "json"), (nz, "npz")):
+        info = preview_velocity_file(f)
+        assert info.is_valid and info.raw_data["format"] == fmt
+        assert (info.n_locations, info.n_time_samples, info.time_range) == (1, 3, (0.0, 2.0))
+        assert info.velocity_range == (1500.0, 2600.0) and info.sample_interval == 1.0
+        np.testing.assert_allclose(velocity_info_to_model(info).data, vm.data)
+
+    one = tmp_path / "one.txt"
+    one.write_text("1500\n1600\n")
+    i3 = preview_velocity_file(one)
+    assert not i3.is_valid and i3.format is VelocityFileFormat.UNKNOWN
+
 def test_models_from_picks():
     from seisproc_amd.utils.anisotropy_io import create_anisotropy_from_picks
     from seisproc_amd.utils.velocity_io import create_velocity_from_picks
