@@ -127,6 +127,27 @@ def test_engine_velocity_model_known_answers():
     assert abs(g.rms_velocity_at_time(1000.0) - math.sqrt(2000.0**2 + 2000.0 * 0.5 * z + (0.5 * z) ** 2 / 3)) < 1e-6
 
 
+
+def test_depth_interval_model_to_vrms_closed_forms():
+    # depth-domain interval velocities -> v_rms at two-way times (the engine
+    # model's table, reference processors/migration/velocity_model.py:234-261)
+    two = VelocityModel(data=np.array([2000.0, 2000.0, 3000.0, 3000.0], np.float32),
+                        z_axis=np.array([0.0, 1000.0, 1000.001, 2000.0]), is_time=False)
+    t1, t2 = 0.5, 1000.0 / 3000.0  # one-way times of the two layers
+    want = math.sqrt((2000.0 ** 2 * t1 + 3000.0 ** 2 * t2) / (t1 + t2))
+    got = two.rms_velocity_at_time(np.array([0.0, 2 * t1, 2 * (t1 + t2)]))
+    np.testing.assert_allclose(got, [2000.0, 2000.0, want], rtol=2e-6)
+    # v(z) = v0 + k z: one-way time t = ln(1 + k z / v0) / k, v(t) = v0 e^(k t),
+    # v_rms^2 = v0^2 (e^(2 k t) - 1) / (2 k t)
+    v0, k = 1800.0, 0.6
+    z = np.arange(0.0, 4000.0 + 0.5, 0.5)
+    lin = VelocityModel(data=(v0 + k * z).astype(np.float32), z_axis=z, is_time=False)
+    t = np.array([0.3, 1.0, 1.4])  # the profile ends at t = ln(1 + k 4000 / v0) / k = 1.41 s
+    want = v0 * np.sqrt(np.expm1(2 * k * t) / (2 * k * t))
+    np.testing.assert_allclose(lin.rms_velocity_at_time(2 * t), want, rtol=1e-6)
+    # v_rms, not the interval velocity v0 e^(k t) at that time (3280 m/s at 1 s)
+    assert v0 * math.exp(k * 1.0) - lin.rms_velocity_at_time(np.array([2.0]))[0] > 700.0
+
 def test_velocity_model_types_and_dix_roundtrip():
     assert VelocityModel(2000.0).is_constant
     m = create_linear_gradient_velocity(1500.0, 600.0, 3.0, 0.002)
