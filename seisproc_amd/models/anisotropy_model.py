@@ -136,9 +136,12 @@ class AnisotropyModel:
                    metadata=d.get("metadata", {}))
 
     def get_summary(self) -> str:
-        if self.anisotropy_type is AnisotropyType.CONSTANT:
-            return f"VTI: epsilon={self.epsilon:.3f}, delta={self.delta:.3f}, eta={self.eta:.3f}"
-        return f"VTI ({self.anisotropy_type.value}), eta range [{np.min(self.eta):.3f}, {np.max(self.eta):.3f}]"
+        """The model's kind and its Thomsen parameters (value, or min..max)."""
+        def span(a):
+            a = np.asarray(a, dtype=np.float64)
+            return f"{float(a.min()):g}" if a.min() == a.max() else f"{float(a.min()):g}..{float(a.max()):g}"
+        return (f"{self.anisotropy_type.value} VTI model: epsilon {span(self.epsilon)}, delta {span(self.delta)}, "
+                f"eta {span(self.eta)}")
 
 
 def create_isotropic() -> AnisotropyModel:

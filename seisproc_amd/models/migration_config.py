@@ -180,13 +180,19 @@ class MigrationConfig:
         return math.radians(self.max_angle_deg)
 
     def get_summary(self) -> str:
-        aniso = f"\n  Anisotropy: VTI ({self.anisotropy_method.value})" if self.use_anisotropy else ""
+        """One ``name: value`` line per setting of the migration."""
         g = self.output_grid
-        return (f"Migration Config:\n  Output: {g.n_inline}x{g.n_xline} traces, {g.n_time} samples\n"
-                f"  Aperture: {self.max_aperture_m}m, max angle {self.max_angle_deg}°\n"
-                f"  Offset range: {self.min_offset_m}-{self.max_offset_m}m\n"
-                f"  Traveltime: {self.traveltime_mode.value}\n  Weights: {self.weight_mode.value}\n"
-                f"  Antialiasing: {'enabled' if self.antialias_enabled else 'disabled'}{aniso}")
+        rows = [
+            ("output", f"{g.n_inline} x {g.n_xline} columns x {g.n_time} samples"),
+            ("aperture", f"{self.max_aperture_m:g} m, angle {self.max_angle_deg:g} deg"),
+            ("offsets", f"{self.min_offset_m:g}..{self.max_offset_m:g} m"),
+            ("traveltime", self.traveltime_mode.value),
+            ("weights", self.weight_mode.value),
+            ("antialias", "on" if self.antialias_enabled else "off"),
+        ]
+        if self.use_anisotropy:
+            rows.append(("anisotropy", f"VTI, {self.anisotropy_method.value}"))
+        return "PSTM configuration\n" + "\n".join(f"  {k:<10} {v}" for k, v in rows)
 
     def to_dict(self) -> Dict[str, Any]:
         return {

@@ -78,11 +78,13 @@ class VelocityModel:
         return torch.as_tensor(self._rms_np(t_ms), dtype=torch.float32, device=device)
 
     def get_summary(self) -> str:
+        """The model's kind and its defining numbers."""
         if self.velocity_type == "constant":
-            return f"Constant velocity: {self.v0:.0f} m/s"
+            return f"velocity {self.v0:g} m/s everywhere"
         if self.velocity_type == "gradient":
-            return f"Linear gradient: v(z) = {self.v0:.0f} + {self.gradient:.4f}*z m/s"
-        return f"From file: {len(self.velocity_array)} points, v0={self.velocity_array[0]:.0f} m/s"
+            return f"velocity {self.v0:g} m/s at the surface, increasing {self.gradient:g} m/s per m"
+        return (f"velocity table of {len(self.velocity_array)} depths, "
+                f"{float(self.velocity_array[0]):g} m/s at the top")
 
 
 def create_velocity_model(velocity_type: str, v0: float, gradient: float = 0.0,

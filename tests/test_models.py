@@ -77,6 +77,21 @@ def test_adapter_summary_names_each_parameter():
     for word in ("grid", "origin", "velocity", "aperture"):
         assert word in s
 
+
+def test_model_summaries():
+    # the reference's anisotropy summary checks (tests/test_anisotropy_model.py:
+    # 315-331): the kind is named and the constant's epsilon is shown
+    from seisproc_amd.models.anisotropy_model import AnisotropyModel, create_constant_anisotropy
+    s = create_constant_anisotropy(0.2, 0.1).get_summary()
+    assert "constant" in s and "0.2" in s
+    z = np.array([0.0, 1.0, 2.0])
+    assert "v_of_z" in AnisotropyModel(epsilon=0.1 + 0.05 * z, delta=0.05 + 0.02 * z, z_axis=z).get_summary()
+    from seisproc_amd.models.migration_config import MigrationConfig
+    c = MigrationConfig().get_summary().lower()
+    for word in ("aperture", "offsets", "traveltime", "weights", "antialias"):
+        assert word in c
+    assert "2500" in engine_vm.VelocityModel("constant", 2500.0).get_summary()
+
 # ---------------------------------------------------------------- MigrationConfig
 def test_migration_config_validation_and_roundtrip():
     with pytest.raises(ValueError, match="max_angle_deg"):
