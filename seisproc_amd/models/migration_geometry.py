@@ -119,5 +119,5 @@ class MigrationGeometry:
 
     def __repr__(self) -> str:
         st = self.get_statistics()
-        return (f"MigrationGeometry(n_traces={self.n_traces}, n_sources={st['n_unique_sources']}, "
-                f"offset=[{st['offset_range'][0]:.0f}, {st['offset_range'][1]:.0f}]m)")
+        lo, hi = st["offset_range"]
+        return f"<MigrationGeometry {self.n_traces} traces from {st['n_unique_sources']} sources, offsets {lo:g}..{hi:g} m>"

@@ -241,11 +241,11 @@ class VelocityModel:
 
     def __repr__(self) -> str:
         if self._type is VelocityType.CONSTANT:
-            return f"VelocityModel(constant={self.data} m/s)"
+            return f"<VelocityModel constant {self.data:g} m/s>"
         if self._type is VelocityType.V_OF_Z:
-            return (f"VelocityModel(v_of_z, n={len(self.data)}, v0={self.v0:.0f} m/s, "
-                    f"gradient={self.gradient:.2f} {self.gradient_unit})")
-        return f"VelocityModel({self._type.value}, shape={self.data.shape})"
+            return (f"<VelocityModel v_of_z, {len(self.data)} samples in {self.z_unit}, {self.v0:g} m/s at the top, "
+                    f"mean gradient {self.gradient or 0.0:.3g} {self.gradient_unit}>")
+        return f"<VelocityModel {self._type.value} {'x'.join(map(str, self.data.shape))}>"
 
 
 def _multilinear(data: np.ndarray, axes_pts) -> np.ndarray:

@@ -69,7 +69,7 @@ class BaseMigrator(ABC):
                 "max_aperture": self.config.max_aperture_m, "max_angle": self.config.max_angle_deg}
 
     def __repr__(self) -> str:
-        return f"{self.__class__.__name__}(velocity={self.velocity_model}, aperture={self.config.max_aperture_m}m)"
+        return f"<{self.__class__.__name__}: {self.velocity_model!r}, aperture {self.config.max_aperture_m:g} m>"
 
 
 class MigrationResult:
@@ -134,4 +134,4 @@ class MigrationResult:
                 "statistics": self.get_statistics()}
 
     def __repr__(self) -> str:
-        return f"MigrationResult(shape={self.shape}, memory={self.memory_gb():.2f}GB)"
+        return f"<MigrationResult {'x'.join(map(str, self.shape))} ({self.memory_gb():.3g} GB)>"
