@@ -24,8 +24,9 @@ and is bound by VALU issue, so the primary figure is VALU lane operations per
 second (counter totals of one call x this run's kernel time) against the FP32
 vector peak; the HBM rate of the algorithmic bytes, the measured HBM traffic
 and the LDS traffic are reported beside it.  The counter totals come from
-the profiles/*.md summary whose source digest matches this build
-(``source_sha()``); without one the counter figures are null.
+the profiles/*.md summary whose source digest matches the kernel sources
+and flags of the library being run (``source_sha()``); without one the
+counter figures are null.
 ``end_to_end`` times one engine-style call from host memory (copy, plan,
 quantisation, forward), outside the timed steps.  ``--vt`` runs the
 precision form with v_rms = 1500 + 600 t.
@@ -56,10 +57,11 @@ HBM_PEAK_GBS = 8000.0
 # instruction per 2 cycles per SIMD at 2.4 GHz on 1024 SIMDs.
 VALU_PEAK_TLANE = 157.3 / 2.0
 # That figure counts packed FP32 (two values per lane per instruction), which
-# this build never emits.  Unpacked FP32 VALU instructions issue at 16 lanes
-# per clock per SIMD -- the lane rate of the published FP64 vector peak, 78.6
-# TFLOP/s -- so the VALU ceiling of these kernels is half the published FP32
-# figure: 39.32 T lane-op/s (1024 SIMDs x 16 lanes x 2.4 GHz).  The issue-rate
+# the kernels here never emit (packed FP32 is switched off at compile time).
+# Unpacked FP32 VALU instructions issue at 16 lanes per clock per SIMD -- the
+# lane rate of the published FP64 vector peak, 78.6 TFLOP/s -- so the VALU
+# ceiling of these kernels is half the published FP32 figure: 39.32 T
+# lane-op/s (1024 SIMDs x 16 lanes x 2.4 GHz).  The issue-rate
 # probe (seisproc_amd/csrc/valu_rates.hip) measures v_fma_f32 streams there.
 VALU_UNPACKED_TLANE = 1024 * 16 * 2.4e9 / 1e12
 # LDS: 256 CUs x 128 bytes per clock x 2.4 GHz.
@@ -86,7 +88,7 @@ def source_sha() -> str:
 
 def load_counters(workload: str, form: str, classes, any_classes: bool = False):
     """Per-call counter totals of this workload and trace form from the JSON
-    block of a profiles/*.md summary stamped with this build's source digest.
+    block of a profiles/*.md summary stamped with the current source digest.
     any_classes: also accept a summary of the same workload on another number
     of offset classes (the caller scales its totals by contributions; the
     record keeps its own "classes").  Returns (record, path) or (None, reason)."""
@@ -120,7 +122,7 @@ def roofline(contrib: float, kernel_s: float, bytes_per_contribution: int, cnt):
     operations per second against the FP32 vector peak; the HBM rate of the
     algorithmic bytes (every tap from HBM) and the measured HBM and LDS
     traffic are reported beside it.  Counter figures are per call, from the
-    profile summary of this build; rates use this run's kernel time."""
+    profile summary of the current sources; rates use this run's kernel time."""
     algo = contrib * bytes_per_contribution / kernel_s / 1e9
     r = {"bound": "valu", "achieved": None, "peak": round(VALU_PEAK_TLANE, 2),
          "unit": "T lane-op/s (FP32 VALU; 157.3 TFLOP/s = 78.65 T lane-FMA/s)", "frac": None, "traffic": None,
