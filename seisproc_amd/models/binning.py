@@ -2,7 +2,8 @@
 
 Same classes, fields, membership rules and JSON form as models/binning.py of
 the reference (``OffsetAzimuthBin`` :34-190, ``BinningTable`` :194-497 and the
-common-offset / uniform / full-stack factories :502-700): a trace belongs to
+common-offset / uniform / OVT / narrow-azimuth / full-stack factories
+:502-700, whose bin names name the per-bin output folders): a trace belongs to
 a bin when offset_min <= offset < offset_max and its azimuth lies in
 [azimuth_min, azimuth_max), the azimuth range wrapping through north when
 azimuth_max < azimuth_min.  The job driver migrates the enabled bins one at
@@ -199,3 +200,30 @@ def create_uniform_offset_binning(offset_min: float = 0.0, offset_max: float = 5
 def create_full_stack_binning(offset_max: float = 10000.0) -> BinningTable:
     return BinningTable(name="Full Stack", bins=[OffsetAzimuthBin("full_stack", 0.0, offset_max)],
                         preset=BinningPreset.FULL_STACK)
+
+
+def create_ovt_binning(offset_ranges: List[Tuple[float, float]], n_azimuth_sectors: int = 4) -> BinningTable:
+    """Offset-vector tiles: every offset range split into n equal azimuth
+    sectors starting at north, named ``ovt_off{min:04d}_az{sector start:03d}``."""
+    step = 360.0 / n_azimuth_sectors
+    bins = [OffsetAzimuthBin(name=f"ovt_off{int(lo):04d}_az{int(k * step):03d}", offset_min=lo, offset_max=hi,
+                             azimuth_min=k * step, azimuth_max=(k + 1) * step)
+            for lo, hi in offset_ranges for k in range(n_azimuth_sectors)]
+    return BinningTable(name="OVT Binning", bins=bins, preset=BinningPreset.OVT,
+                        metadata={"n_azimuth_sectors": n_azimuth_sectors})
+
+
+def create_narrow_azimuth_binning(offset_min: float = 0.0, offset_max: float = 5000.0, inline_azimuth: float = 0.0,
+                                  azimuth_width: float = 20.0) -> BinningTable:
+    """Four sectors of azimuth_width centred on the inline direction, its
+    opposite and the two crossline directions (``inline_pos``, ``inline_neg``,
+    ``xline_pos``, ``xline_neg``); sectors across north wrap, and a sector
+    ending at north ends at 360 (the bin's upper bound must be positive)."""
+    az0 = inline_azimuth % 360.0
+    h = 0.5 * azimuth_width
+    bins = [OffsetAzimuthBin(name=name, offset_min=offset_min, offset_max=offset_max,
+                             azimuth_min=(az0 + turn - h) % 360.0, azimuth_max=(az0 + turn + h) % 360.0 or 360.0)
+            for name, turn in (("inline_pos", 0.0), ("inline_neg", 180.0), ("xline_pos", 90.0),
+                               ("xline_neg", 270.0))]
+    return BinningTable(name="Narrow Azimuth Binning", bins=bins, preset=BinningPreset.NARROW_AZIMUTH,
+                        metadata={"inline_azimuth": az0, "azimuth_width": azimuth_width})

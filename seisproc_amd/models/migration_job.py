@@ -1,9 +1,11 @@
-"""Migration job configuration (reference models/migration_job.py:27-334).
+"""Migration job configuration (reference models/migration_job.py:27-491).
 
-Same fields, defaults, validation messages, JSON save/load and derived
-objects (get_output_grid / get_migration_config / get_velocity_model).  The
-binning table is carried as its serialised dict (trace binning for the job
-driver; the HIP plan bins traces on the GPU itself).  get_velocity_model
+Same fields, defaults, validation messages, JSON save/load, derived objects
+(get_output_grid / get_migration_config / get_velocity_model /
+get_binning_table) and job templates.  The binning table is a BinningTable
+(saved as its dict form); a preset name resolves to the preset's table (the
+job driver's trace binning; the HIP plan bins traces on the GPU itself).
+get_velocity_model
 loads velocity_file through VelocityModel.load (the reference calls a load
 its VelocityModel does not define, :235).
 """
@@ -16,6 +18,7 @@ from datetime import datetime
 from pathlib import Path
 from typing import Any, Dict, List, Optional
 
+from .binning import BinningTable
 from .migration_config import InterpolationMode, MigrationConfig, OutputGrid, TraveltimeMode, WeightMode
 from .velocity_model import VelocityModel
 
@@ -75,7 +78,7 @@ class MigrationJobConfig:
     near_offset_taper_m: float = 0.0
     anti_alias_filter: bool = True
     binning_preset: Optional[str] = None
-    binning_table: Optional[Dict[str, Any]] = None
+    binning_table: Optional[BinningTable] = None
     header_mapping: HeaderMapping = field(default_factory=HeaderMapping)
     processing_mode: str = "sequential"
     max_workers: int = 1
@@ -154,11 +157,13 @@ class MigrationJobConfig:
                                  z_axis=t, gradient=self.velocity_gradient, v0=self.velocity_v0)
         return VelocityModel(data=float(self.velocity_v0))
 
-    def get_binning_table(self) -> Dict[str, Any]:
+    def get_binning_table(self) -> BinningTable:
+        """The explicit table, else a fresh table of the named preset."""
         if self.binning_table is not None:
             return self.binning_table
         if self.binning_preset:
-            return {"preset": self.binning_preset}
+            from ..utils.binning_presets import get_preset
+            return get_preset(self.binning_preset)
         raise ValueError("No binning configuration available")
 
     def to_dict(self) -> Dict[str, Any]:
@@ -172,7 +177,7 @@ class MigrationJobConfig:
             "create_fold_volume", "compress_output", "created_at", "description", "metadata")}
         d["header_mapping"] = self.header_mapping.to_dict()
         if self.binning_table is not None:
-            d["binning_table"] = self.binning_table
+            d["binning_table"] = self.binning_table.to_dict()
         return d
 
     @classmethod
@@ -180,7 +185,7 @@ class MigrationJobConfig:
         d = dict(d)
         hm = HeaderMapping.from_dict(d.pop("header_mapping", {}) or {})
         bt = d.pop("binning_table", None)
-        return cls(header_mapping=hm, binning_table=bt,
+        return cls(header_mapping=hm, binning_table=BinningTable.from_dict(bt) if bt is not None else None,
                    **{k: v for k, v in d.items() if k in cls.__dataclass_fields__})
 
     def save(self, filepath: str) -> None:
@@ -192,3 +197,61 @@ class MigrationJobConfig:
     def load(cls, filepath: str) -> "MigrationJobConfig":
         with open(filepath) as f:
             return cls.from_dict(json.load(f))
+
+
+# ---------------------------------------------------------------------------
+# job templates (reference models/migration_job.py:341-491): a preset binning
+# and survey-typical velocity, aperture and angle per survey kind
+
+_TEMPLATES: Dict[str, Dict[str, Any]] = {
+    "land_3d": dict(name="Land 3D Migration", binning_preset="land_3d", max_aperture_m=3000.0, max_angle_deg=60.0,
+                    description="Standard land 3D survey migration with 10 offset bins"),
+    "marine": dict(name="Marine Migration", binning_preset="marine", max_aperture_m=5000.0, max_angle_deg=50.0,
+                   description="Marine streamer migration with 6 offset bins"),
+    "wide_azimuth_ovt": dict(name="Wide Azimuth OVT Migration", binning_preset="wide_azimuth_ovt",
+                             max_aperture_m=4000.0, max_angle_deg=65.0,
+                             description="Wide-azimuth migration with 16 OVT bins (4 offset x 4 azimuth)"),
+    "full_stack": dict(name="Full Stack Migration", binning_preset="full_stack", max_aperture_m=5000.0,
+                       max_angle_deg=70.0, create_stack_volume=True,
+                       description="Single full-stack migration output"),
+}
+
+
+def _from_template(kind: str, input_file: str, output_directory: str, **values) -> MigrationJobConfig:
+    return MigrationJobConfig(input_file=input_file, output_directory=output_directory, **_TEMPLATES[kind], **values)
+
+
+def create_land_3d_template(input_file: str, output_directory: str, velocity_v0: float = 3000.0) -> MigrationJobConfig:
+    return _from_template("land_3d", input_file, output_directory, velocity_v0=velocity_v0)
+
+
+def create_marine_template(input_file: str, output_directory: str, velocity_v0: float = 1500.0,
+                           velocity_gradient: float = 0.3) -> MigrationJobConfig:
+    return _from_template("marine", input_file, output_directory, velocity_v0=velocity_v0,
+                          velocity_gradient=velocity_gradient)
+
+
+def create_wide_azimuth_template(input_file: str, output_directory: str,
+                                 velocity_v0: float = 2800.0) -> MigrationJobConfig:
+    return _from_template("wide_azimuth_ovt", input_file, output_directory, velocity_v0=velocity_v0)
+
+
+def create_full_stack_template(input_file: str, output_directory: str,
+                               velocity_v0: float = 2500.0) -> MigrationJobConfig:
+    return _from_template("full_stack", input_file, output_directory, velocity_v0=velocity_v0)
+
+
+JOB_TEMPLATES = {"land_3d": create_land_3d_template, "marine": create_marine_template,
+                 "wide_azimuth_ovt": create_wide_azimuth_template, "full_stack": create_full_stack_template}
+
+
+def list_templates() -> List[str]:
+    return list(JOB_TEMPLATES)
+
+
+def create_from_template(template_name: str, input_file: str, output_directory: str, **kwargs) -> MigrationJobConfig:
+    """The named template; keyword arguments go to its factory (velocity_v0,
+    and velocity_gradient for marine)."""
+    if template_name not in JOB_TEMPLATES:
+        raise ValueError(f"Unknown template '{template_name}'. Available: {', '.join(JOB_TEMPLATES)}")
+    return JOB_TEMPLATES[template_name](input_file, output_directory, **kwargs)

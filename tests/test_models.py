@@ -250,6 +250,76 @@ def test_job_config_roundtrip(tmp_path):
     assert j3.get_velocity_model().v0 == 1500.0
 
 
+
+def test_job_config_binning_validation_and_templates(tmp_path):
+    # the reference's job-config tests restated (tests/test_migration_job.py:
+    # 95-410): validation messages, the gradient velocity, preset and explicit
+    # binning tables, serialisation of a table, and the templates
+    from seisproc_amd.models.binning import create_uniform_offset_binning
+    from seisproc_amd.models import migration_job as mj
+    inp = tmp_path / "in.sgy"
+    inp.write_bytes(b"dummy")
+    out = str(tmp_path / "out")
+    ok = MigrationJobConfig(name="v", input_file=str(inp), output_directory=out, binning_preset="land_3d")
+    assert ok.validate() == [] and ok.is_valid() and ok.velocity_v0 == 2500.0 and ok.created_at
+    bad = lambda **kw: MigrationJobConfig(name="b", input_file=str(inp), output_directory=out, **kw).validate()
+    assert any("binning" in e.lower() for e in bad())
+    assert any("velocity" in e.lower() for e in bad(binning_preset="land_3d", velocity_v0=-100.0))
+    assert any("time" in e.lower() for e in bad(binning_preset="land_3d", time_min_ms=5000.0, time_max_ms=2000.0))
+    m = MigrationJobConfig(name="m", input_file=str(inp), output_directory=out, binning_preset="land_3d",
+                           max_aperture_m=2500.0, max_angle_deg=55.0, velocity_v0=3000.0, velocity_gradient=0.5)
+    assert (m.get_migration_config().max_aperture_m, m.get_migration_config().max_angle_deg) == (2500.0, 55.0)
+    vm = m.get_velocity_model()
+    assert vm.v0 == 3000.0 and vm.gradient == 0.5
+    table = m.get_binning_table()
+    assert len(table.bins) == 10 and table.bins[0].name == "land3d_0000_0200" and table.offset_range == (0, 5000)
+    custom = create_uniform_offset_binning(0, 3000, 3)
+    custom.name = "Custom 3 Bins"
+    c = MigrationJobConfig(name="c", input_file=str(inp), output_directory=out, binning_table=custom)
+    assert c.get_binning_table() is custom
+    back = MigrationJobConfig.from_dict(json.loads(json.dumps(c.to_dict())))
+    assert back.binning_table.name == "Custom 3 Bins" and len(back.binning_table.bins) == 3
+    assert set(mj.list_templates()) == {"land_3d", "marine", "wide_azimuth_ovt", "full_stack"}
+    land = mj.create_land_3d_template(str(inp), out)
+    assert land.binning_preset == "land_3d" and land.max_aperture_m == 3000.0 and "Land 3D" in land.name
+    marine = mj.create_from_template("marine", str(inp), out)
+    assert marine.binning_preset == "marine" and marine.velocity_v0 == 1500.0 and marine.velocity_gradient == 0.3
+    assert mj.create_from_template("land_3d", str(inp), out, velocity_v0=3500.0).velocity_v0 == 3500.0
+    assert mj.create_full_stack_template(str(inp), out).create_stack_volume is True
+    assert mj.create_wide_azimuth_template(str(inp), out).get_binning_table().n_bins == 16
+    with pytest.raises(ValueError, match="Unknown template"):
+        mj.create_from_template("nonexistent", str(inp), out)
+
+
+def test_binning_factories_and_presets():
+    # the reference's binning tests restated (tests/test_binning.py:440-540)
+    from seisproc_amd.models.binning import (BinningPreset, create_full_stack_binning, create_narrow_azimuth_binning,
+                                             create_ovt_binning)
+    from seisproc_amd.utils.binning_presets import get_preset, get_preset_description, list_presets
+    ovt = create_ovt_binning([(0, 500), (500, 1000)], n_azimuth_sectors=4)
+    assert ovt.n_bins == 8 and ovt.preset is BinningPreset.OVT
+    assert all(abs(b.azimuth_width - 90.0) < 0.1 for b in ovt)
+    assert ovt.assign_trace(700.0, 95.0) == ["ovt_off0500_az090"]
+    nar = create_narrow_azimuth_binning(0.0, 5000.0, inline_azimuth=0.0, azimuth_width=20.0)
+    assert nar.n_bins == 4 and nar.preset is BinningPreset.NARROW_AZIMUTH
+    assert {"inline_pos", "xline_pos"} <= {b.name for b in nar}
+    # the inline sector wraps through north; a sector ending at north ends at 360
+    assert nar.assign_trace(100.0, 355.0) == ["inline_pos"] and nar.assign_trace(100.0, 5.0) == ["inline_pos"]
+    assert nar.assign_trace(100.0, 45.0) == []
+    tilted = create_narrow_azimuth_binning(inline_azimuth=350.0, azimuth_width=20.0)
+    assert tilted.get_bin("inline_pos").azimuth_max == 360.0 and tilted.assign_trace(10.0, 345.0) == ["inline_pos"]
+    fs = create_full_stack_binning(offset_max=5000.0)
+    assert fs.n_bins == 1 and fs.bins[0].offset_max == 5000.0 and fs.bins[0].is_full_azimuth
+    assert {"land_3d", "marine", "wide_azimuth_ovt"} <= set(list_presets())
+    assert get_preset("land_3d").preset is BinningPreset.COMMON_OFFSET and get_preset("land_3d").n_bins == 10
+    assert get_preset("marine").n_bins == 6 and get_preset("marine").offset_range == (0, 8000)
+    assert get_preset("wide_azimuth_ovt").preset is BinningPreset.OVT and get_preset("wide_azimuth_ovt").n_bins == 16
+    assert get_preset("narrow_azimuth").n_bins == 4 and get_preset("full_stack").n_bins == 1
+    with pytest.raises(ValueError):
+        get_preset("nonexistent")
+    assert "land" in get_preset_description("land_3d").lower()
+    assert get_preset_description("nonexistent") == "No description available"
+
 def test_seismic_data_contract():
     d = SeismicData(traces=np.zeros((11, 3), np.int16), sample_rate=2.0)
     assert d.traces.dtype == np.float32 and d.n_samples == 11 and d.duration == 20.0 and d.nyquist_freq == 250.0
