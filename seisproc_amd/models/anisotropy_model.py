@@ -1,4 +1,4 @@
-"""VTI anisotropy model (reference models/anisotropy_model.py:23-366).
+"""VTI anisotropy model (reference models/anisotropy_model.py:23-596).
 
 Thomsen epsilon / delta with eta = (epsilon - delta) / (1 + 2 delta)
 (:133-155).  ``eta_on_grid`` evaluates eta at every output sample of an
@@ -44,11 +44,11 @@ class AnisotropyModel:
                                     3: AnisotropyType.V_3D}[self.epsilon.ndim]
         else:
             self.anisotropy_type = AnisotropyType.CONSTANT
-        if self.eta is None:
-            self.eta = self.compute_eta(self.epsilon, self.delta)
         if np.asarray(self.epsilon).shape != np.asarray(self.delta).shape:
             raise ValueError(f"epsilon shape {np.asarray(self.epsilon).shape} must match delta shape "
                              f"{np.asarray(self.delta).shape}")
+        if self.eta is None:
+            self.eta = self.compute_eta(self.epsilon, self.delta)
         self._check_bounds()
 
     def _check_bounds(self):
@@ -157,3 +157,58 @@ def create_eta_field(eta: np.ndarray, x_axis, y_axis, t_axis) -> AnisotropyModel
     eta = np.asarray(eta, dtype=np.float64)
     return AnisotropyModel(epsilon=eta, delta=np.zeros_like(eta), eta=eta, x_axis=np.asarray(x_axis),
                            y_axis=np.asarray(y_axis), z_axis=np.asarray(t_axis))
+
+
+# Typical shale (epsilon, delta) by intensity (reference models/anisotropy_model.py:395-421)
+SHALE_PARAMETERS = {"weak": (0.1, 0.05), "moderate": (0.2, 0.1), "strong": (0.35, 0.15)}
+
+
+def create_shale_anisotropy(intensity: str = "moderate") -> AnisotropyModel:
+    if intensity not in SHALE_PARAMETERS:
+        raise ValueError(f"intensity must be one of {list(SHALE_PARAMETERS)}")
+    eps, dl = SHALE_PARAMETERS[intensity]
+    return AnisotropyModel(epsilon=eps, delta=dl, metadata={"formation": "shale", "intensity": intensity})
+
+
+def create_1d_anisotropy(z_axis, epsilon, delta) -> AnisotropyModel:
+    """epsilon and delta tabulated along z_axis (two-way time for the kernels)."""
+    return AnisotropyModel(epsilon=np.asarray(epsilon, dtype=np.float64), delta=np.asarray(delta, dtype=np.float64),
+                           z_axis=np.asarray(z_axis, dtype=np.float64))
+
+
+def create_gradient_anisotropy(epsilon_surface: float, epsilon_gradient: float, delta_surface: float,
+                               delta_gradient: float, z_max: float, dz: float = 0.01) -> AnisotropyModel:
+    """epsilon and delta growing linearly from their surface values, sampled
+    every dz on [0, z_max]."""
+    z = np.arange(0.0, z_max + dz, dz)
+    m = create_1d_anisotropy(z, epsilon_surface + epsilon_gradient * z, delta_surface + delta_gradient * z)
+    m.metadata.update(epsilon_surface=epsilon_surface, epsilon_gradient=epsilon_gradient,
+                      delta_surface=delta_surface, delta_gradient=delta_gradient)
+    return m
+
+
+def _v_factor(v0, f):
+    # v0 sqrt(f) with f floored at 0.01, as the reference guards unphysical input
+    return v0 * np.sqrt(np.maximum(f, 0.01))
+
+
+def compute_vti_phase_velocity(v0: float, theta, epsilon, delta):
+    """Weak-anisotropy phase velocity at angle theta (radians) from vertical:
+    v0 sqrt(1 + 2 delta sin^2 cos^2 + 2 epsilon sin^4)."""
+    s2 = np.sin(theta) ** 2
+    return _v_factor(v0, 1.0 + 2.0 * np.asarray(delta) * s2 * (1.0 - s2) + 2.0 * np.asarray(epsilon) * s2 * s2)
+
+
+def compute_nmo_velocity(v0: float, delta):
+    """Short-spread moveout velocity v0 sqrt(1 + 2 delta)."""
+    return _v_factor(v0, 1.0 + 2.0 * np.asarray(delta))
+
+
+def compute_horizontal_velocity(v0: float, epsilon):
+    """Horizontal P velocity v0 sqrt(1 + 2 epsilon)."""
+    return _v_factor(v0, 1.0 + 2.0 * np.asarray(epsilon))
+
+
+def compute_effective_eta(epsilon, delta):
+    """eta = (epsilon - delta) / (1 + 2 delta)."""
+    return AnisotropyModel.compute_eta(epsilon, delta)

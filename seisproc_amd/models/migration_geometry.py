@@ -1,4 +1,4 @@
-"""Survey geometry (reference models/migration_geometry.py:19-402).
+"""Survey geometry (reference models/migration_geometry.py:19-551).
 
 Same fields and derived attributes as the reference, except that the
 coordinates stay FP64: the reference casts them to FP32 on construction
@@ -9,7 +9,7 @@ GPU, once per trace.
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import Any, Dict, Optional
+from typing import Any, Dict, Optional, Tuple
 
 import numpy as np
 
@@ -85,13 +85,87 @@ class MigrationGeometry:
         return {"x_min": float(xs.min()), "x_max": float(xs.max()), "y_min": float(ys.min()),
                 "y_max": float(ys.max())}
 
+    # -- per-trace positions
+    def get_source_position(self, idx: int) -> Tuple[float, float]:
+        return float(self.source_x[idx]), float(self.source_y[idx])
+
+    def get_receiver_position(self, idx: int) -> Tuple[float, float]:
+        return float(self.receiver_x[idx]), float(self.receiver_y[idx])
+
+    def get_cdp_position(self, idx: int) -> Tuple[float, float]:
+        return float(self.cdp_x[idx]), float(self.cdp_y[idx])
+
+    # -- extents
+    @staticmethod
+    def _span(a: np.ndarray) -> Tuple[float, float]:
+        return float(np.min(a)), float(np.max(a))
+
+    def get_survey_bounds(self) -> Dict[str, Tuple[float, float]]:
+        """(min, max) of x and y over sources and receivers, of offset and of azimuth."""
+        b = self.get_bounds()
+        return {"x": (b["x_min"], b["x_max"]), "y": (b["y_min"], b["y_max"]),
+                "offset": self._span(self.offset), "azimuth": self._span(self.azimuth)}
+
+    def get_offset_range(self) -> Tuple[float, float]:
+        return self._span(self.offset)
+
+    def get_azimuth_range(self) -> Tuple[float, float]:
+        return self._span(self.azimuth)
+
+    # -- offset / azimuth selections (inclusive bounds; an azimuth range with
+    # min > max wraps through north, e.g. 350..10)
+    def _bin_mask(self, min_offset=None, max_offset=None, min_azimuth=None, max_azimuth=None) -> np.ndarray:
+        keep = np.ones(self.n_traces, dtype=bool)
+        if min_offset is not None:
+            off = self.offset
+            keep &= (off >= min_offset) & (off <= max_offset)
+        if min_azimuth is not None:
+            az = self.azimuth
+            if min_azimuth <= max_azimuth:
+                keep &= (az >= min_azimuth) & (az <= max_azimuth)
+            else:
+                keep &= (az >= min_azimuth) | (az <= max_azimuth)
+        return keep
+
+    def filter_by_offset(self, min_offset: float, max_offset: float) -> "MigrationGeometry":
+        return self.select(self._bin_mask(min_offset, max_offset))
+
+    def filter_by_azimuth(self, min_azimuth: float, max_azimuth: float) -> "MigrationGeometry":
+        return self.select(self._bin_mask(min_azimuth=min_azimuth, max_azimuth=max_azimuth))
+
+    def filter_by_offset_azimuth(self, min_offset: float, max_offset: float, min_azimuth: float,
+                                 max_azimuth: float) -> "MigrationGeometry":
+        return self.select(self._bin_mask(min_offset, max_offset, min_azimuth, max_azimuth))
+
+    def get_indices_for_bin(self, min_offset: float, max_offset: float, min_azimuth: float,
+                            max_azimuth: float) -> np.ndarray:
+        return np.nonzero(self._bin_mask(min_offset, max_offset, min_azimuth, max_azimuth))[0]
+
+    # -- stations
+    @staticmethod
+    def _unique_xy(x: np.ndarray, y: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+        if not len(x):
+            return x[:0], y[:0]
+        u = np.unique(np.stack([x, y], 1), axis=0)
+        return u[:, 0], u[:, 1]
+
+    def get_unique_sources(self) -> Tuple[np.ndarray, np.ndarray]:
+        return self._unique_xy(self.source_x, self.source_y)
+
+    def get_unique_receivers(self) -> Tuple[np.ndarray, np.ndarray]:
+        return self._unique_xy(self.receiver_x, self.receiver_y)
+
     def get_statistics(self) -> Dict[str, Any]:
+        """Trace and station counts, extents and offset moments (the
+        reference's keys, :308-329)."""
+        if not self.n_traces:
+            return {"n_traces": 0, "n_unique_sources": 0, "n_unique_receivers": 0, "offset_range": (0.0, 0.0)}
+        b = self.get_survey_bounds()
         off = self.offset
-        return {"n_traces": self.n_traces,
-                "n_unique_sources": int(len(np.unique(np.stack([self.source_x, self.source_y], 1), axis=0)))
-                if self.n_traces else 0,
-                "offset_range": (float(off.min()), float(off.max())) if self.n_traces else (0.0, 0.0),
-                "bounds": self.get_bounds() if self.n_traces else None}
+        return {"n_traces": self.n_traces, "n_unique_sources": len(self.get_unique_sources()[0]),
+                "n_unique_receivers": len(self.get_unique_receivers()[0]), "x_range": b["x"], "y_range": b["y"],
+                "offset_range": b["offset"], "offset_mean": float(off.mean()), "offset_std": float(off.std()),
+                "azimuth_range": b["azimuth"]}
 
     def to_dict(self) -> Dict[str, Any]:
         d = {"source_x": self.source_x.tolist(), "source_y": self.source_y.tolist(),
@@ -121,3 +195,38 @@ class MigrationGeometry:
         st = self.get_statistics()
         lo, hi = st["offset_range"]
         return f"<MigrationGeometry {self.n_traces} traces from {st['n_unique_sources']} sources, offsets {lo:g}..{hi:g} m>"
+
+
+def create_synthetic_geometry(n_shots: int = 10, n_receivers_per_shot: int = 100, shot_spacing: float = 100.0,
+                              receiver_spacing: float = 25.0, near_offset: float = 100.0,
+                              shot_line_azimuth: float = 0.0) -> MigrationGeometry:
+    """Streamer-style line (reference :409-471): shots every shot_spacing
+    along the azimuth from the origin, each recorded by a receiver line
+    trailing behind it from near_offset every receiver_spacing."""
+    u = np.array([np.sin(np.radians(shot_line_azimuth)), np.cos(np.radians(shot_line_azimuth))])
+    shot = np.arange(n_shots, dtype=np.float64)[:, None] * shot_spacing
+    trail = near_offset + np.arange(n_receivers_per_shot, dtype=np.float64)[None, :] * receiver_spacing
+    s = np.broadcast_to(shot, (n_shots, n_receivers_per_shot)).ravel()
+    r = (shot - trail).ravel()
+    return MigrationGeometry(s * u[0], s * u[1], r * u[0], r * u[1],
+                             metadata={"synthetic": True, "n_shots": n_shots,
+                                       "n_receivers_per_shot": n_receivers_per_shot,
+                                       "shot_spacing": shot_spacing, "receiver_spacing": receiver_spacing})
+
+
+def create_land_3d_geometry(n_source_lines: int = 5, n_sources_per_line: int = 20, n_receiver_lines: int = 10,
+                            n_receivers_per_line: int = 50, source_line_spacing: float = 400.0,
+                            source_spacing: float = 50.0, receiver_line_spacing: float = 200.0,
+                            receiver_spacing: float = 25.0) -> MigrationGeometry:
+    """Orthogonal land patch (reference :474-540): source lines along x
+    (spaced in y), receiver lines along y (spaced in x), every source
+    recorded by every receiver (source-major trace order)."""
+    sx, sy = np.meshgrid(np.arange(n_sources_per_line) * source_spacing,
+                         np.arange(n_source_lines) * source_line_spacing)
+    rx, ry = np.meshgrid(np.arange(n_receiver_lines) * receiver_line_spacing,
+                         np.arange(n_receivers_per_line) * receiver_spacing, indexing="ij")
+    ns, nr = sx.size, rx.size
+    return MigrationGeometry(np.repeat(sx.ravel(), nr), np.repeat(sy.ravel(), nr), np.tile(rx.ravel(), ns),
+                             np.tile(ry.ravel(), ns),
+                             metadata={"synthetic": True, "geometry_type": "land_3d_orthogonal",
+                                       "n_source_lines": n_source_lines, "n_receiver_lines": n_receiver_lines})

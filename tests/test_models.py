@@ -203,6 +203,38 @@ def test_eta_formula_and_interpolation():
         AnisotropyModel(0.1, -0.6)
 
 
+
+def test_anisotropy_factories_and_vti_velocities():
+    # the reference's anisotropy tests restated (tests/test_anisotropy_model.py:
+    # 110-280) plus identities that pin the relations independently
+    from seisproc_amd.models import anisotropy_model as am
+    m = am.create_1d_anisotropy(np.array([0.0, 1.0, 2.0]), np.array([0.1, 0.2, 0.3]), np.array([0.05, 0.1, 0.15]))
+    assert m.anisotropy_type is am.AnisotropyType.V_OF_Z
+    e, d, _ = m.get_parameters_at(0.5)
+    assert abs(e - 0.15) < 1e-12 and abs(d - 0.075) < 1e-12
+    g = am.create_gradient_anisotropy(0.1, 0.1, 0.05, 0.05, z_max=2.0, dz=1.0)
+    assert g.anisotropy_type is am.AnisotropyType.V_OF_Z and g.epsilon[0] == 0.1
+    assert abs(g.get_parameters_at(1.0)[0] - 0.2) < 1e-12
+    assert [(s.epsilon, s.delta) for s in map(am.create_shale_anisotropy, ("weak", "moderate", "strong"))] == \
+        [(0.1, 0.05), (0.2, 0.1), (0.35, 0.15)]
+    with pytest.raises(ValueError):
+        am.create_shale_anisotropy("invalid")
+    with pytest.raises(ValueError, match="must match"):
+        am.AnisotropyModel(epsilon=np.array([0.1, 0.2]), delta=np.array([0.05, 0.1, 0.15]))
+    with pytest.raises(ValueError, match="Unstable"):
+        am.AnisotropyModel(epsilon=0.0, delta=-0.6)
+    v0, eps, dl = 2500.0, 0.2, 0.1
+    assert abs(am.compute_vti_phase_velocity(v0, 0.0, eps, dl) - v0) < 1e-9
+    vh = am.compute_horizontal_velocity(v0, eps)
+    assert abs(am.compute_vti_phase_velocity(v0, np.pi / 2, eps, dl) - vh) < 1e-9
+    vn = am.compute_nmo_velocity(v0, dl)
+    # 1 + 2 eta = (v_horizontal / v_nmo)^2
+    assert abs(1 + 2 * am.compute_effective_eta(eps, dl) - (vh / vn) ** 2) < 1e-12
+    # weak anisotropy: the phase velocity is Thomsen's linear form to second order
+    th = np.linspace(0.0, np.pi / 2, 7)
+    lin = v0 * (1 + 1e-3 * np.sin(th) ** 2 * np.cos(th) ** 2 + 2e-3 * np.sin(th) ** 4)
+    np.testing.assert_allclose(am.compute_vti_phase_velocity(v0, th, 2e-3, 1e-3), lin, rtol=1e-5)
+
 def test_eta_field_on_grid():
     from seisproc_amd.abi_types import PstmParams
     p = PstmParams.from_dict(dict(n_il=3, n_xl=4, n_tau=5, dtau=0.5, il_dx=10.0, xl_dy=10.0))
@@ -319,6 +351,34 @@ def test_binning_factories_and_presets():
         get_preset("nonexistent")
     assert "land" in get_preset_description("land_3d").lower()
     assert get_preset_description("nonexistent") == "No description available"
+
+
+def test_geometry_queries_and_synthetic_layouts():
+    # the reference's geometry tests restated (tests/test_migration_models.py:
+    # 330-355) plus the selections' bounds and azimuth wrap
+    from seisproc_amd.models.migration_geometry import (MigrationGeometry, create_land_3d_geometry,
+                                                        create_synthetic_geometry)
+    g = create_synthetic_geometry(n_shots=5, n_receivers_per_shot=24, receiver_spacing=25.0, near_offset=100.0)
+    assert g.n_traces == 120 and g.offset[0] == pytest.approx(100.0)
+    assert g.get_offset_range() == pytest.approx((100.0, 100.0 + 23 * 25.0))
+    assert np.allclose(g.azimuth, 180.0)  # receivers trail south of shots on a north-going line
+    st = g.get_statistics()
+    assert (st["n_unique_sources"], st["n_unique_receivers"]) == (5, 5 * 24 - 4 * 20)
+    assert {"x_range", "y_range", "offset_mean", "offset_std", "azimuth_range"} <= set(st)
+    land = create_land_3d_geometry(n_source_lines=2, n_sources_per_line=3, n_receiver_lines=2,
+                                   n_receivers_per_line=5)
+    assert land.n_traces == 60 and land.get_source_position(5) == (0.0, 0.0) and land.get_source_position(10) == (50.0, 0.0)
+    assert land.get_receiver_position(7) == (200.0, 50.0)
+    # azimuths 0, 90, 180, 270, 350 degrees at offsets 100..500 m
+    az = np.radians([0.0, 90.0, 180.0, 270.0, 350.0])
+    off = np.array([100.0, 200.0, 300.0, 400.0, 500.0])
+    z = np.zeros(5)
+    q = MigrationGeometry(z, z, off * np.sin(az), off * np.cos(az))
+    assert q.get_indices_for_bin(150.0, 400.0, 0.0, 360.0).tolist() == [1, 2, 3]  # inclusive
+    assert q.filter_by_azimuth(340.0, 10.0).trace_indices.tolist() == [0, 4]      # wraps through north
+    assert q.filter_by_offset_azimuth(0.0, 450.0, 80.0, 200.0).trace_indices.tolist() == [1, 2]
+    assert q.get_cdp_position(1) == pytest.approx((100.0, 0.0))
+    assert q.get_survey_bounds()["offset"] == pytest.approx((100.0, 500.0))
 
 def test_seismic_data_contract():
     d = SeismicData(traces=np.zeros((11, 3), np.int16), sample_rate=2.0)
