@@ -9,6 +9,8 @@ from __future__ import annotations
 
 from typing import Callable, Dict, List, Tuple
 
+import numpy as np
+
 from ..models.binning import (BinningTable, create_common_offset_binning, create_full_stack_binning,
                               create_narrow_azimuth_binning, create_ovt_binning)
 
@@ -73,3 +75,32 @@ def get_preset(name: str, **kwargs) -> BinningTable:
 
 def get_preset_description(name: str) -> str:
     return _PRESETS[name][1] if name in _PRESETS else "No description available"
+
+
+def create_custom_offset_binning(offset_min: float, offset_max: float, n_bins: int,
+                                 logarithmic: bool = False) -> BinningTable:
+    """n_bins full-azimuth offset bins named ``custom_*``: equal widths, or
+    with logarithmic=True geometric edges from max(1, offset_min) to
+    offset_max (narrower near offsets), the first edge reset to offset_min."""
+    if logarithmic:
+        lo = max(1.0, offset_min)
+        edges = [lo * (offset_max / lo) ** (i / n_bins) for i in range(n_bins + 1)]
+        edges[0] = offset_min
+    else:
+        w = (offset_max - offset_min) / n_bins
+        edges = [offset_min + i * w for i in range(n_bins + 1)]
+    return create_common_offset_binning(list(zip(edges[:-1], edges[1:])), name_prefix="custom")
+
+
+def suggest_binning(offsets, azimuths, target_traces_per_bin: int = 10000) -> BinningTable:
+    """Full-azimuth offset bins holding about target_traces_per_bin traces
+    each: max(1, n // target) bins with edges at equally spaced percentiles
+    of the offsets (the azimuths do not shape the suggestion)."""
+    off = np.asarray(offsets, dtype=np.float64)
+    n_bins = max(1, off.size // target_traces_per_bin)
+    edges = np.percentile(off, np.linspace(0.0, 100.0, n_bins + 1))
+    table = create_common_offset_binning([(float(a), float(b)) for a, b in zip(edges[:-1], edges[1:])],
+                                         name_prefix="suggested")
+    table.name = f"Suggested - {n_bins} Bins"
+    table.metadata.update(source="auto_suggested", target_traces_per_bin=target_traces_per_bin)
+    return table

@@ -321,6 +321,28 @@ def write_velocity_auto(model: VelocityModel, filepath: Union[str, Path], **kwar
         write_velocity_json(model, path, **kwargs)
 
 
+def convert_velocity_file(input_path: Union[str, Path], output_path: Union[str, Path], input_type: str = "rms",
+                          output_type: str = "interval", **read_kwargs) -> VelocityModel:
+    """Read a 1D velocity function, convert between RMS and interval
+    velocities (Dix, on the file's own time axis) and write it
+    (reference utils/velocity_io.py:902-950); the output format follows the
+    output path's extension.  Returns the written model."""
+    from ..models.velocity_model import interval_to_rms_velocity, rms_to_interval_velocity
+    kinds = ("rms", "interval")
+    if input_type not in kinds or output_type not in kinds:
+        raise ValueError(f"velocity types are 'rms' or 'interval', got {input_type!r} -> {output_type!r}")
+    model = read_velocity_auto(input_path, velocity_type=input_type, **read_kwargs)
+    if model.velocity_type is not VelocityType.V_OF_Z:
+        raise ValueError("Velocity conversion only supported for 1D v(z) models")
+    if input_type != output_type:
+        conv = rms_to_interval_velocity if input_type == "rms" else interval_to_rms_velocity
+        t_axis, v = conv(model.z_axis, model.data)
+        model = VelocityModel(data=np.asarray(v, dtype=np.float32), z_axis=np.asarray(t_axis), is_time=model.is_time,
+                              metadata={**model.metadata, "velocity_type": output_type})
+    write_velocity_auto(model, output_path)
+    return model
+
+
 # ---------------------------------------------------------------------------
 # SEG-Y, preview and picks
 

@@ -2,6 +2,7 @@
 CDP-time-velocity, inline-xline-time-velocity, JSON and .npz layouts) and
 their sampling onto the image grid."""
 import json
+import math
 
 import numpy as np
 import pytest
@@ -248,6 +249,41 @@ def test_preview_and_info_to_model_tables_json_npz(tmp_path):
     one.write_text("1500\n1600\n")
     i3 = preview_velocity_file(one)
     assert not i3.is_valid and i3.format is VelocityFileFormat.UNKNOWN
+
+
+def test_convert_velocity_file_rms_interval_round_trip(tmp_path):
+    # the reference's conversion test restated (tests/test_velocity_io.py:
+    # 300-330), then the reverse conversion recovers the RMS function
+    from seisproc_amd.utils.velocity_io import convert_velocity_file, read_velocity_auto
+    rms = tmp_path / "rms.txt"
+    rms.write_text("0\t2000\n500\t2200\n1000\t2500\n1500\t2800\n")
+    itv = tmp_path / "interval.txt"
+    m = convert_velocity_file(str(rms), str(itv), input_type="rms", output_type="interval", time_unit="ms")
+    assert itv.exists() and m.metadata.get("velocity_type") == "interval" and abs(m.data[0] - 2000.0) < 1.0
+    # Dix: the second interval carries sqrt((2200^2 0.5 - 2000^2 0) / 0.5) = 2200, the third
+    # sqrt((2500^2 1.0 - 2200^2 0.5) / 0.5)
+    assert abs(m.data[2] - math.sqrt((2500.0 ** 2 - 2200.0 ** 2 * 0.5) / 0.5)) < 0.05
+    back = tmp_path / "back.json"
+    r = convert_velocity_file(str(itv), str(back), input_type="interval", output_type="rms", time_unit="s")
+    np.testing.assert_allclose(r.data, [2000.0, 2200.0, 2500.0, 2800.0], rtol=1e-5)
+    assert read_velocity_auto(back).metadata["velocity_type"] == "rms"
+    with pytest.raises(ValueError):
+        convert_velocity_file(str(rms), str(itv), input_type="rms", output_type="average")
+
+
+def test_custom_and_suggested_offset_binning():
+    # the reference's checks (tests/test_binning.py:538-560) and the edges
+    from seisproc_amd.utils.binning_presets import create_custom_offset_binning, suggest_binning
+    t = create_custom_offset_binning(0.0, 2000.0, 4)
+    assert t.n_bins == 4 and [b.offset_max for b in t] == [500.0, 1000.0, 1500.0, 2000.0]
+    lg = create_custom_offset_binning(0.0, 1000.0, 3, logarithmic=True)
+    np.testing.assert_allclose([b.offset_max for b in lg], [10.0, 100.0, 1000.0])
+    assert lg.bins[0].offset_min == 0.0
+    off = np.random.default_rng(3).uniform(0, 3000, 1000)
+    s = suggest_binning(off, np.zeros(1000), target_traces_per_bin=200)
+    assert s.n_bins == 5 and s.name == "Suggested - 5 Bins"
+    counts = [int(b.contains_batch(off, np.zeros(1000)).sum()) for b in s]
+    assert sum(counts) == 999 and max(counts) - min(counts) <= 2  # the largest offset sits on the last upper edge
 
 def test_models_from_picks():
     from seisproc_amd.utils.anisotropy_io import create_anisotropy_from_picks
