@@ -149,6 +149,48 @@ class BinningTable:
                             enabled_only: bool = True) -> Dict[str, np.ndarray]:
         return {b.name: b.contains_batch(offsets, azimuths) for b in (self.enabled_bins if enabled_only else self.bins)}
 
+    # -- table checks (reference :327-438)
+    def check_coverage(self, offsets: np.ndarray, azimuths: np.ndarray) -> Dict[str, Any]:
+        """Traces per enabled bin, traces in no bin and in several bins."""
+        masks = self.assign_traces_batch(offsets, azimuths)
+        n = len(offsets)
+        hits = np.zeros(n, dtype=np.int64)
+        for m in masks.values():
+            hits += m
+        unassigned = int(np.count_nonzero(hits == 0))
+        return {"n_traces": n, "n_unassigned": unassigned, "n_multi_bin": int(np.count_nonzero(hits > 1)),
+                "bin_counts": {k: int(m.sum()) for k, m in masks.items()},
+                "coverage_percent": 100.0 * (n - unassigned) / n if n else 0.0}
+
+    @staticmethod
+    def _arcs(b: OffsetAzimuthBin) -> List[Tuple[float, float]]:
+        return [(b.azimuth_min, 360.0), (0.0, b.azimuth_max)] if b.wraps else [(b.azimuth_min, b.azimuth_max)]
+
+    def check_overlaps(self) -> List[Tuple[str, str]]:
+        """Pairs of bins whose [offset) ranges and azimuth sectors (wrapping
+        through north where they do) intersect in an interval."""
+        out = []
+        for i, a in enumerate(self.bins):
+            for b in self.bins[i + 1:]:
+                if max(a.offset_min, b.offset_min) >= min(a.offset_max, b.offset_max):
+                    continue
+                if any(max(p, r) < min(q, s) for p, q in self._arcs(a) for r, s in self._arcs(b)):
+                    out.append((a.name, b.name))
+        return out
+
+    def check_gaps(self, offset_step: float = 100.0) -> List[Tuple[float, float]]:
+        """Offset intervals inside the table's offset range that no bin
+        covers.  The intervals are exact; offset_step (the reference's scan
+        resolution) is accepted and not needed."""
+        if not self.bins:
+            return []
+        gaps, reach = [], None
+        for lo, hi in sorted((b.offset_min, b.offset_max) for b in self.bins):
+            if reach is not None and lo > reach:
+                gaps.append((reach, lo))
+            reach = hi if reach is None else max(reach, hi)
+        return gaps
+
     def __iter__(self) -> Iterator[OffsetAzimuthBin]:
         return iter(self.bins)
 

@@ -24,7 +24,8 @@ class BaseMigrator(ABC):
     def __init__(self, velocity_model, config: MigrationConfig, **kwargs):
         self.velocity_model = velocity_model
         self.config = config
-        self.migrator_params = kwargs
+        # a migrator that is also a processor has set its processor params already
+        self.params = {**getattr(self, "params", {}), **kwargs}
         self._progress_callback: Optional[ProgressCallback] = None
         self._validate_inputs()
         logger.info("Initialized %s", self.__class__.__name__)
@@ -67,6 +68,14 @@ class BaseMigrator(ABC):
         return {"class": self.__class__.__name__, "velocity_type": str(vt),
                 "output_grid": f"{g.n_inline}x{g.n_xline}x{g.n_time}",
                 "max_aperture": self.config.max_aperture_m, "max_angle": self.config.max_angle_deg}
+
+    def to_dict(self) -> Dict[str, Any]:
+        """What rebuilds this migrator: class, module, velocity model,
+        configuration and keyword parameters."""
+        vm = self.velocity_model
+        return {"class_name": self.__class__.__name__, "module": self.__class__.__module__,
+                "velocity_model": vm.to_dict() if hasattr(vm, "to_dict") else vm,
+                "config": self.config.to_dict(), "params": dict(self.params)}
 
     def __repr__(self) -> str:
         return f"<{self.__class__.__name__}: {self.velocity_model!r}, aperture {self.config.max_aperture_m:g} m>"

@@ -119,7 +119,12 @@ class KirchhoffMigrator(BaseMigrator, BaseProcessor):
         return proc.process(data)
 
     def to_dict(self) -> Dict[str, Any]:
-        return BaseProcessor.to_dict(self)
+        """The processor form (class_name, module, params: what
+        BaseProcessor.from_dict rebuilds the migrator from) with the
+        migrator form's velocity_model and config beside it."""
+        d = BaseProcessor.to_dict(self)
+        d.update(velocity_model=self.velocity.to_dict(), config=self.config.to_dict())
+        return d
 
 
 def create_kirchhoff_migrator(velocity, config: MigrationConfig, device=None, **kw) -> KirchhoffMigrator:
@@ -171,6 +176,7 @@ class OptimizedKirchhoffMigrator(KirchhoffMigrator):
                           gpu_tiling=bool(enable_gpu_tiling))
         self._stats = OptimizationStats()
         self._profile_tiles = False
+        self._bin_traces = None
         super().__init__(velocity, config, device=device, **kw)
 
     def migrate_gather(self, gather: SeismicData, geometry: MigrationGeometry,
@@ -187,6 +193,18 @@ class OptimizedKirchhoffMigrator(KirchhoffMigrator):
                                         memory_peak_mb=torch.cuda.max_memory_allocated(self.device) / 2**20,
                                         time_accumulation=time.time() - t0)
         return image, fold
+
+    def initialize_for_bin(self, all_geometry: MigrationGeometry) -> None:
+        """Start a bin made of several gathers (reference :260-284, where it
+        builds the bin's trace spatial index once).  Each call's HIP plan
+        bins its own traces by midpoint on the GPU, so nothing is prebuilt;
+        the bin's trace count is kept for the statistics."""
+        self._bin_traces = int(all_geometry.n_traces)
+
+    def reset_for_new_bin(self) -> None:
+        """Forget the previous bin and its statistics (reference :286-293)."""
+        self._bin_traces = None
+        self._stats = OptimizationStats()
 
     def get_optimization_stats(self) -> OptimizationStats:
         """Statistics of the last migrate_gather call."""

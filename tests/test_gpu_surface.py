@@ -169,6 +169,12 @@ def test_optimized_migrator_factory_equals_class_api():
         st = mig.get_optimization_stats()
         assert st.total_traces == s.n_traces and st.traces_processed == s.n_traces
         assert "GPUTiling=" + str(level == "high") in mig.get_description()
+        # bin lifecycle and serialisation of the interface
+        mig.initialize_for_bin(geo)
+        mig.reset_for_new_bin()
+        assert mig.get_optimization_stats().total_traces == 0
+        d = mig.to_dict()
+        assert d["class_name"] == "OptimizedKirchhoffMigrator" and d["config"] == cfg.to_dict()
     # SPREADING weight against the oracle (raw image and fold, [tau][il][xl])
     ref, rfold = oracle.forward(p.replace(weight_mode=1), s.traces.double().numpy(), sx, sy, rx, ry,
                                 np.full(p.n_tau, 2000.0))

@@ -323,6 +323,23 @@ def test_job_config_binning_validation_and_templates(tmp_path):
         mj.create_from_template("nonexistent", str(inp), out)
 
 
+
+def test_binning_table_checks():
+    # the reference's checks restated (tests/test_binning.py:321-367), plus
+    # an azimuth overlap that only part of the sectors share and exact gaps
+    from seisproc_amd.models.binning import BinningTable, OffsetAzimuthBin as B
+    t = BinningTable(name="t", bins=[B("a", 0.0, 500.0), B("b", 500.0, 1000.0)])
+    c = t.check_coverage(np.array([100.0, 600.0, 1500.0, 300.0]), np.zeros(4))
+    assert (c["n_traces"], c["n_unassigned"], c["n_multi_bin"], c["coverage_percent"]) == (4, 1, 0, 75.0)
+    assert c["bin_counts"] == {"a": 2, "b": 1}
+    assert t.check_overlaps() == []
+    assert BinningTable(bins=[B("a", 0.0, 600.0), B("b", 400.0, 1000.0)]).check_overlaps() == [("a", "b")]
+    sectors = BinningTable(bins=[B("n", 0.0, 100.0, 350.0, 20.0), B("e", 0.0, 100.0, 15.0, 90.0),
+                                 B("s", 0.0, 100.0, 100.0, 200.0)])
+    assert sectors.check_overlaps() == [("n", "e")]  # 15..20 degrees, across the wrap of "n"
+    g = BinningTable(bins=[B("a", 0.0, 300.0), B("b", 500.0, 800.0), B("c", 700.0, 900.0), B("d", 950.0, 1000.0)])
+    assert g.check_gaps(offset_step=50.0) == [(300.0, 500.0), (900.0, 950.0)]
+
 def test_binning_factories_and_presets():
     # the reference's binning tests restated (tests/test_binning.py:440-540)
     from seisproc_amd.models.binning import (BinningPreset, create_full_stack_binning, create_narrow_azimuth_binning,
@@ -379,6 +396,32 @@ def test_geometry_queries_and_synthetic_layouts():
     assert q.filter_by_offset_azimuth(0.0, 450.0, 80.0, 200.0).trace_indices.tolist() == [1, 2]
     assert q.get_cdp_position(1) == pytest.approx((100.0, 0.0))
     assert q.get_survey_bounds()["offset"] == pytest.approx((100.0, 500.0))
+
+
+def test_base_migrator_params_and_serialisation():
+    # a BaseMigrator keeps its keyword parameters (reference base_migrator.py:
+    # 24-66, 230-243) without replacing params a processor base has set
+    from seisproc_amd.processors.migration.base_migrator import BaseMigrator
+
+    class Probe(BaseMigrator):
+        def _validate_inputs(self): pass
+        def migrate_gather(self, gather, geometry): pass
+        def migrate_dataset(self, gathers, geometries, batch_size=1): pass
+        def estimate_memory_gb(self, n_traces, n_samples): return 1.0
+        def get_description(self): return "probe"
+
+    cfg = MigrationConfig()
+    m = Probe(VelocityModel(2000.0), cfg, chunk=7)
+    d = m.to_dict()
+    assert d["class_name"] == "Probe" and d["params"] == {"chunk": 7} and d["config"] == cfg.to_dict()
+    assert d["velocity_model"]["data"] == 2000.0
+
+    class WithParams(Probe):
+        def __init__(self):
+            self.params = {"kept": 1}
+            super().__init__(VelocityModel(2000.0), cfg, extra=2)
+
+    assert WithParams().params == {"kept": 1, "extra": 2}
 
 def test_seismic_data_contract():
     d = SeismicData(traces=np.zeros((11, 3), np.int16), sample_rate=2.0)
