@@ -302,6 +302,61 @@ def create_linear_gradient_velocity(v0: float, gradient: float, z_max: float, dz
                          gradient=gradient, v0=v0, metadata={"model_type": "linear_gradient"})
 
 
+def _checked_field(data, axes, names):
+    data = np.asarray(data, dtype=np.float32)
+    if data.ndim != len(axes):
+        raise ValueError(f"Data must be {len(axes)}D, got shape {data.shape}")
+    for k, (ax, nm) in enumerate(zip(axes, names)):
+        if data.shape[k] != len(ax):
+            raise ValueError(f"Data {nm}-dimension ({data.shape[k]}) must match {nm}_axis length ({len(ax)})")
+    return data
+
+
+def create_2d_velocity(data, z_axis, x_axis, is_time: bool = True,
+                       metadata: Optional[Dict[str, Any]] = None) -> VelocityModel:
+    """v(x, z) from data (n_z, n_x) on its axes (z two-way time in s when is_time)."""
+    data = _checked_field(data, (z_axis, x_axis), ("z", "x"))
+    return VelocityModel(data=data, z_axis=np.asarray(z_axis, dtype=np.float64),
+                         x_axis=np.asarray(x_axis, dtype=np.float64), is_time=is_time,
+                         metadata=metadata or {"model_type": "2d_field"})
+
+
+def create_3d_velocity(data, z_axis, x_axis, y_axis, is_time: bool = True,
+                       metadata: Optional[Dict[str, Any]] = None) -> VelocityModel:
+    """v(x, y, z) from data (n_z, n_x, n_y) on its axes."""
+    data = _checked_field(data, (z_axis, x_axis, y_axis), ("z", "x", "y"))
+    return VelocityModel(data=data, z_axis=np.asarray(z_axis, dtype=np.float64),
+                         x_axis=np.asarray(x_axis, dtype=np.float64), y_axis=np.asarray(y_axis, dtype=np.float64),
+                         is_time=is_time, metadata=metadata or {"model_type": "3d_cube"})
+
+
+def create_2d_gradient_velocity(v0: float, z_gradient: float, x_gradient: float, z_max: float, x_max: float,
+                                dz: float = 0.004, dx: float = 25.0, is_time: bool = True) -> VelocityModel:
+    """v(x, z) = v0 + z_gradient z + x_gradient x sampled every dz, dx from the origin."""
+    z = np.arange(0, z_max + dz, dz, dtype=np.float64)
+    x = np.arange(0, x_max + dx, dx, dtype=np.float64)
+    return VelocityModel(data=(v0 + z_gradient * z[:, None] + x_gradient * x[None, :]).astype(np.float32),
+                         z_axis=z, x_axis=x, is_time=is_time,
+                         metadata={"model_type": "2d_linear_gradient", "v0": v0, "z_gradient": z_gradient,
+                                   "x_gradient": x_gradient})
+
+
+def create_layered_velocity(layer_depths, layer_velocities, z_max: float, dz: float = 0.004,
+                            is_time: bool = True) -> VelocityModel:
+    """Piecewise-constant v(z): layer i holds from layer_depths[i] (its top)
+    to the next top, the last layer to z_max; samples above the first top
+    are 0 (the first top is meant to be 0)."""
+    tops = np.asarray(layer_depths, dtype=np.float64)
+    vel = np.asarray(layer_velocities, dtype=np.float64)
+    if len(tops) != len(vel):
+        raise ValueError("layer_depths and layer_velocities must have same length")
+    z = np.arange(0, z_max + dz, dz, dtype=np.float64)
+    k = np.searchsorted(tops, z, side="right") - 1  # the layer whose top is at or above z
+    data = np.where(k >= 0, vel[np.clip(k, 0, None)], 0.0).astype(np.float32)
+    return VelocityModel(data=data, z_axis=z, is_time=is_time,
+                         metadata={"model_type": "layered", "n_layers": len(vel)})
+
+
 def create_from_rms_velocity(t_axis, v_rms, is_time: bool = True) -> VelocityModel:
     return VelocityModel(data=np.asarray(v_rms, dtype=np.float32), z_axis=np.asarray(t_axis, dtype=np.float64),
                          is_time=is_time, metadata={"velocity_type": "rms"})

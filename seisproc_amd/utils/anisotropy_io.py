@@ -15,11 +15,11 @@ from __future__ import annotations
 import json
 import logging
 from pathlib import Path
-from typing import Optional, Union
+from typing import Dict, Optional, Union
 
 import numpy as np
 
-from ..models.anisotropy_model import AnisotropyModel, AnisotropyType
+from ..models.anisotropy_model import AnisotropyModel, AnisotropyType, create_1d_anisotropy
 
 logger = logging.getLogger(__name__)
 
@@ -176,3 +176,23 @@ def create_anisotropy_from_picks(picks, dz: float = 0.004, z_max: Optional[float
                         z_axis=z)
     m.metadata.update(source="picks", n_picks=len(picks))
     return m
+
+
+def estimate_anisotropy_from_wells(well_data: Dict[str, np.ndarray], method: str = "average") -> AnisotropyModel:
+    """A v(z) anisotropy model from a well (reference utils/anisotropy_io.py:
+    438-486): the logged epsilon and delta along 'z' where given, else, with
+    only velocities ('vp'), the typical shale values epsilon 0.15 and
+    delta 0.08 at every z.  The method name is recorded in the metadata."""
+    z = well_data.get("z")
+    if z is None:
+        raise ValueError("Well data must include 'z' axis")
+    z = np.asarray(z, dtype=np.float64)
+    if "epsilon" in well_data and "delta" in well_data:
+        eps, dl = well_data["epsilon"], well_data["delta"]
+    elif well_data.get("vp") is not None:
+        eps, dl = np.full(z.shape, 0.15), np.full(z.shape, 0.08)
+    else:
+        raise ValueError("Well data must include either (epsilon, delta) or velocities")
+    model = create_1d_anisotropy(z, eps, dl)
+    model.metadata.update(source="well_estimate", method=method)
+    return model

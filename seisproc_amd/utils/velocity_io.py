@@ -551,3 +551,43 @@ def create_velocity_from_picks(picks: List[Tuple[float, float]], dt: float = 0.0
     t_axis = dt * np.arange(n)
     return VelocityModel(data=np.interp(t_axis, tp, vp).astype(np.float32), z_axis=t_axis, is_time=True,
                          metadata={"velocity_type": velocity_type, "source": "picks", "n_picks": len(picks)})
+
+
+def get_velocity_at_cdp(model: VelocityModel, cdp: int, times: Optional[np.ndarray] = None):
+    """(times, velocities) of the model's velocity function at a CDP
+    (reference utils/velocity_io.py:1291-1330): constant and v(t) models
+    have one function (a constant one is given on 0 and 6 s unless times are
+    asked for); a v(t, CDP) table is interpolated to the CDP number."""
+    t = model.velocity_type
+    if t is VelocityType.CONSTANT:
+        times = np.array([0.0, 6.0], dtype=np.float32) if times is None else np.asarray(times)
+        return times, np.full(np.shape(times), model.data, dtype=np.float32)
+    if t is VelocityType.V_OF_Z:
+        if times is None:
+            return model.z_axis, model.data
+        return times, np.interp(times, model.z_axis, model.data).astype(np.float32)
+    if t is VelocityType.V_OF_XZ:
+        times = model.z_axis if times is None else np.asarray(times)
+        return times, model.get_velocity_at(times, x=float(cdp))
+    raise ValueError(f"Unsupported velocity type: {t}")
+
+
+def get_velocity_summary(model: VelocityModel) -> str:
+    """A few lines describing the model: kind, extents, velocity range,
+    velocity type and source file where known."""
+    t = model.velocity_type
+    rows = [("kind", t.value)]
+    if t is VelocityType.CONSTANT:
+        rows.append(("velocity", f"{float(model.data):g} m/s"))
+    else:
+        d = np.asarray(model.data)
+        rows.append(("samples", " x ".join(map(str, d.shape))))
+        rows.append(("times" if model.is_time else "depths", f"{float(model.z_axis[0]):g}..{float(model.z_axis[-1]):g}"))
+        if model.x_axis is not None:
+            rows.append(("x", f"{float(model.x_axis[0]):g}..{float(model.x_axis[-1]):g}"))
+        rows.append(("velocity", f"{float(d.min()):g}..{float(d.max()):g} m/s"))
+    if model.metadata.get("velocity_type"):
+        rows.append(("type", str(model.metadata["velocity_type"])))
+    if model.metadata.get("source_file"):
+        rows.append(("source", Path(model.metadata["source_file"]).name))
+    return "velocity model\n" + "\n".join(f"  {k:<9} {v}" for k, v in rows)

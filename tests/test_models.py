@@ -179,6 +179,30 @@ def test_velocity_model_types_and_dix_roundtrip():
     np.testing.assert_allclose(d.data, m.data)
 
 
+
+def test_lateral_and_layered_velocity_factories():
+    # 2D/3D fields (z first), the 2D gradient and layered v(z) (reference
+    # models/velocity_model.py:516-700)
+    from seisproc_amd.models.velocity_model import (VelocityType, create_2d_gradient_velocity, create_2d_velocity,
+                                                    create_3d_velocity, create_layered_velocity)
+    z, x, y = np.array([0.0, 1.0, 2.0]), np.array([0.0, 100.0]), np.array([0.0, 50.0, 100.0, 150.0])
+    m2 = create_2d_velocity(np.array([[1500, 1600], [2000, 2100], [2500, 2600]]), z, x)
+    assert m2.velocity_type is VelocityType.V_OF_XZ and m2.metadata["model_type"] == "2d_field"
+    assert float(m2.get_velocity_at(0.5, 50.0)[0]) == pytest.approx(1800.0)
+    with pytest.raises(ValueError, match="x-dimension"):
+        create_2d_velocity(np.ones((3, 3)) * 2000, z, x)
+    m3 = create_3d_velocity(np.full((3, 2, 4), 2000.0), z, x, y)
+    assert m3.velocity_type is VelocityType.V_OF_XYZ and m3.data.shape == (3, 2, 4)
+    with pytest.raises(ValueError, match="3D"):
+        create_3d_velocity(np.ones((3, 2)), z, x, y)
+    g = create_2d_gradient_velocity(1500.0, 500.0, 0.1, z_max=2.0, x_max=1000.0, dz=0.5, dx=250.0)
+    assert g.data.shape == (5, 5) and g.data[4, 4] == pytest.approx(1500.0 + 500.0 * 2.0 + 0.1 * 1000.0)
+    lay = create_layered_velocity([0.0, 0.5, 1.2], [1500.0, 2200.0, 3000.0], z_max=2.0, dz=0.1)
+    assert lay.metadata["n_layers"] == 3
+    np.testing.assert_array_equal(lay.data[[0, 4, 5, 11, 12, 20]], [1500, 1500, 2200, 2200, 3000, 3000])
+    with pytest.raises(ValueError):
+        create_layered_velocity([0.0, 1.0], [1500.0], z_max=2.0)
+
 def test_bspline_velocity_partition_of_unity_and_fit():
     b = BSplineVelocity(20, 18, 60, shape=(6, 5, 8), device="cpu")
     coef = torch.full((6, 5, 8), 2500.0)

@@ -285,6 +285,30 @@ def test_custom_and_suggested_offset_binning():
     counts = [int(b.contains_batch(off, np.zeros(1000)).sum()) for b in s]
     assert sum(counts) == 999 and max(counts) - min(counts) <= 2  # the largest offset sits on the last upper edge
 
+
+def test_velocity_at_cdp_summary_and_well_anisotropy():
+    # velocity functions by CDP and the summary (reference utils/velocity_io.py:
+    # 1291-1368); anisotropy from a well (utils/anisotropy_io.py:438-486)
+    from seisproc_amd.models.velocity_model import create_2d_velocity
+    from seisproc_amd.utils.anisotropy_io import estimate_anisotropy_from_wells
+    t, v = vio.get_velocity_at_cdp(VelocityModel(2500.0), 7)
+    assert t.tolist() == [0.0, 6.0] and v.tolist() == [2500.0, 2500.0]
+    one = VelocityModel(data=np.array([1500.0, 2500.0], np.float32), z_axis=np.array([0.0, 2.0]))
+    assert vio.get_velocity_at_cdp(one, 3, times=np.array([1.0]))[1].tolist() == [2000.0]
+    line = create_2d_velocity(np.array([[1500.0, 1700.0], [2500.0, 2900.0]]), [0.0, 2.0], [10.0, 20.0])
+    t2, v2 = vio.get_velocity_at_cdp(line, 15)
+    np.testing.assert_allclose(v2, [1600.0, 2700.0])
+    s = vio.get_velocity_summary(line)
+    assert "v_of_xz" in s and "1500..2900 m/s" in s and "10..20" in s
+    assert "2500 m/s" in vio.get_velocity_summary(VelocityModel(2500.0))
+    z = np.array([0.0, 1.0, 2.0])
+    w = estimate_anisotropy_from_wells({"z": z, "epsilon": np.array([0.1, 0.2, 0.3]), "delta": np.array([0.0, 0.05, 0.1])})
+    assert w.metadata["source"] == "well_estimate" and abs(w.get_parameters_at(0.5)[0] - 0.15) < 1e-12
+    e = estimate_anisotropy_from_wells({"z": z, "vp": np.full(3, 3000.0)}, method="empirical")
+    assert np.allclose(e.epsilon, 0.15) and np.allclose(e.delta, 0.08) and e.metadata["method"] == "empirical"
+    with pytest.raises(ValueError):
+        estimate_anisotropy_from_wells({"z": z})
+
 def test_models_from_picks():
     from seisproc_amd.utils.anisotropy_io import create_anisotropy_from_picks
     from seisproc_amd.utils.velocity_io import create_velocity_from_picks
