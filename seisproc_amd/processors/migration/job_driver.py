@@ -32,7 +32,7 @@ import os
 import time
 from dataclasses import dataclass, field
 from pathlib import Path
-from typing import Callable, Dict, Iterator, List, Optional, Tuple
+from typing import Any, Callable, Dict, Iterator, List, Optional, Tuple
 
 import numpy as np
 import torch
@@ -73,6 +73,28 @@ class ArrayBinSource:
 
     def bin_summary(self) -> Dict[str, int]:
         return {b.name: int(self._members[b.name].size) for b in self.binning.enabled_bins}
+
+    # BinnedDataset's queries (reference utils/dataset_indexer.py:677-736)
+    def get_bin_trace_numbers(self, bin_name: str) -> np.ndarray:
+        return self._members.get(bin_name, np.zeros(0, dtype=np.int64))
+
+    def get_bin_trace_count(self, bin_name: str) -> int:
+        return int(self.get_bin_trace_numbers(bin_name).size)
+
+    def get_bin_summary(self) -> Dict[str, int]:
+        return self.bin_summary()
+
+    def get_coverage_report(self) -> Dict[str, Any]:
+        """Traces per bin, assigned in total, in no bin and in several bins."""
+        n = int(self.traces.shape[0])
+        hits = np.zeros(n, dtype=np.int64)
+        for idx in self._members.values():
+            hits[idx] += 1
+        summary = self.bin_summary()
+        unassigned = int(np.count_nonzero(hits == 0))
+        return {"n_traces": n, "n_bins": len(summary), "bin_counts": summary, "total_assigned": sum(summary.values()),
+                "n_unassigned": unassigned, "n_multi_bin": int(np.count_nonzero(hits > 1)),
+                "coverage_percent": 100.0 * (n - unassigned) / n if n else 0.0}
 
     def identity(self) -> str:
         """Digest of the input: every source/receiver coordinate, the trace

@@ -68,6 +68,14 @@ def test_bin_source_assigns_by_offset_and_azimuth():
         b = t.get_bin(name)
         assert np.all((off[idx] >= b.offset_min) & (off[idx] < b.offset_max))
     np.testing.assert_array_equal(src.read(idx), tr[idx])
+    # the BinnedDataset queries (reference utils/dataset_indexer.py:677-736)
+    assert src.get_bin_trace_count("offset_0200_0400") == 0  # disabled: not assigned
+    assert src.get_bin_summary() == {n: int(i.size) for n, i in src.iterate_bins()}
+    rep = src.get_coverage_report()
+    in_enabled = int(np.count_nonzero(((off >= 0) & (off < 200)) | ((off >= 400) & (off < 600))))
+    assert rep["n_traces"] == len(off) and rep["total_assigned"] == in_enabled and rep["n_multi_bin"] == 0
+    assert rep["n_unassigned"] == len(off) - in_enabled
+    np.testing.assert_array_equal(src.get_bin_trace_numbers("offset_0400_0600"), idx)
 
 
 def _read_bin(d, name):
