@@ -335,3 +335,32 @@ def test_antialias_integrated_traces_long_segmented_windows(n_samples, dtau, fla
     ref, rfold = oracle.forward(p, s.traces.double().numpy(), sx, sy, rx, ry, vel.double().numpy(),
                                 eta=None if eta is None else eta.double().numpy())
     _compare(f"integrated-segmented-{n_samples}-flags{flags}", img, fold, ref, rfold)
+
+
+
+@pytest.mark.parametrize("flags", [FLAG_KAHAN, FLAG_AA | FLAG_KAHAN], ids=["plain", "antialias"])
+def test_forward_reciprocity_and_exact_scaling(flags):
+    # Swapping every source with its receiver gives the same fold exactly
+    # (the masks are symmetric in the legs) and the same image to FP32
+    # rounding (the legs are summed in the other order); doubling the traces
+    # doubles the image exactly and leaves the fold unchanged (every step is
+    # linear and a factor of 2 rounds nothing).  Both also hold for the oracle.
+    from seisproc_amd.processors.migration.hip_ops import PstmPlan
+    s = S.tiny_survey(seed=6, n_bins=8, n_classes=2, n_samples=201, n_img=6, aperture=300.0, flags=flags)
+    p = s.params
+    tr, vel = s.traces.to(DEV), s.vel.to(DEV)
+    out = []
+    for src, rcv, d in ((s.src_xy, s.rcv_xy, tr), (s.rcv_xy, s.src_xy, tr), (s.src_xy, s.rcv_xy, 2.0 * tr)):
+        plan = PstmPlan(p, src, rcv)
+        comp = torch.zeros(plan.image_shape, device=DEV)
+        img, fold = plan.forward(d, vel, comp=comp)
+        out.append((img, comp, fold))
+    torch.cuda.synchronize()
+    (img, comp, fold), (img_r, comp_r, fold_r), (img2, comp2, fold2) = out
+    assert float(fold.sum()) > 0
+    assert torch.equal(fold_r, fold) and torch.equal(fold2, fold)
+    tot, tot_r = img.double() + comp.double(), img_r.double() + comp_r.double()
+    err = float((tot_r - tot).norm() / tot.norm())
+    print(f"[reciprocity-{flags}] rel_rms {err:.2e}", flush=True)
+    assert err <= 1e-6
+    assert torch.equal(img2, 2.0 * img) and torch.equal(comp2, 2.0 * comp)

@@ -469,3 +469,34 @@ def test_antialias_zero_length_leg_is_finite(oracle_lib):
     p = _p(n_samples=50, n_tau=3, flags=FLAG_AA, aa_max=8.0)
     img, fold = oracle.forward(p, np.arange(50.0)[None, :], [0.0], [0.0], [0.0], [0.0], np.full(3, 2000.0))
     assert np.all(np.isfinite(img)) and fold[0, 0, 0] == 1
+
+
+
+@pytest.mark.parametrize("flags", [0, FLAG_AA, FLAG_AA | FLAG_VTI | FLAG_TAPER], ids=["plain", "aa", "aa-vti-taper"])
+def test_oracle_reciprocity_and_linearity(oracle_lib, flags):
+    # Source-receiver reciprocity: the DSR time, weights, masks, antialias
+    # half-width and tapers are symmetric in the two legs, so swapping every
+    # source with its receiver leaves image and fold unchanged.  Linearity:
+    # the image is a linear function of the traces (fold does not depend on
+    # them).  A leg-specific mistake (one leg's slope, one leg's taper) breaks
+    # the first; any amplitude-dependent step breaks the second.
+    s = S.tiny_survey(seed=6, n_bins=6, n_classes=2, n_samples=151, n_img=4, aperture=300.0, flags=flags,
+                      lateral=bool(flags & FLAG_VTI))
+    sx, sy, rx, ry = s.numpy_geometry()
+    p = s.params.replace(vel_mode=1 if s.vel.dim() == 3 else 0)
+    if flags & FLAG_TAPER:
+        p = p.replace(min_offset_m=100.0, max_offset_m=700.0, taper_width=0.4)
+    d = s.traces.double().numpy()
+    v = s.vel.double().numpy()
+    eta = s.eta.double().numpy() if s.eta is not None else None
+    img, fold = oracle.forward(p, d, sx, sy, rx, ry, v, eta=eta)
+    assert fold.sum() > 0
+    img_r, fold_r = oracle.forward(p, d, rx, ry, sx, sy, v, eta=eta)
+    np.testing.assert_array_equal(fold_r, fold)
+    scale = np.abs(img).max()
+    assert np.abs(img_r - img).max() <= 1e-12 * scale
+    d2 = np.random.default_rng(7).normal(size=d.shape)
+    img2, fold2 = oracle.forward(p, d2, sx, sy, rx, ry, v, eta=eta)
+    img3, _ = oracle.forward(p, 0.75 * d - 2.5 * d2, sx, sy, rx, ry, v, eta=eta)
+    np.testing.assert_array_equal(fold2, fold)
+    assert np.abs(img3 - (0.75 * img - 2.5 * img2)).max() <= 1e-12 * max(scale, np.abs(img2).max())
